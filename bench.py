@@ -1,0 +1,155 @@
+"""Benchmark of the XPBD Neo-Hookean substep on one MI355X.
+
+Workload (one "step" = one frame of the single-large-mesh configuration):
+  64^3 hex box jittered +-0.2 cell (seed 0), 6 tets/hex -> 274 625 vertices,
+  1 572 864 tets; E=1e6, nu=0.48, density 1000; plane y=0 + fixed sphere r=0.3;
+  dt = 1/60, 10 substeps x 100 Chebyshev-accelerated block Jacobi iterations.
+
+Prints one JSON line: tet-iterations/s over the timed frames (inputs resident
+in HBM), the roofline of the dominant kernel (algorithmic bytes / HIP-event
+duration against 8 TB/s), the per-frame latency percentiles, and the CPU
+oracle timed on the host cores over a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "rdc-xpbd-taichi_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def b_alg_iteration(T: int, V: int) -> int:
+    """Algorithmic bytes of one Jacobi iteration (block + Chebyshev, f32 state,
+    i32 indices, each vertex gathered once): 136 T + 140 V."""
+    return 136 * T + 140 * V
+
+
+def b_alg_project(T: int, V: int) -> int:
+    """Algorithmic bytes of one projection launch: per tet 16 (ids) + 36
+    (inv_Dm) + 4 (V0) + 8 + 8 (lambda read/write); per vertex 16 (x, inv_m
+    gather) + 12 (delta_x write)."""
+    return 72 * T + 28 * V
+
+
+def cpu_baseline(scene, substeps: int, threads: int) -> dict:
+    from oracle import Oracle  # the CPU restatement, used here only as the baseline
+    p = dict(scene.params)
+    o = Oracle(scene.mesh.verts, scene.mesh.tets, scene.sdfs, precision=32, threads=threads,
+               gravity=scene.gravity, **p)
+    o.compute_lumped_mass(scene.total_mass)
+    o.set_log(False)
+    sub_dt = p["dt"] / p["substeps_num"]
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        o.substep(sub_dt)
+    el = time.perf_counter() - t0
+    tit = scene.mesh.n_tets * p["constraints_iter"] * substeps
+    return {"value": tit / el, "unit": "tet-iterations/s", "cores": threads, "kind": "port",
+            "sample": f"{substeps} substeps x {p['constraints_iter']} iterations of the same "
+                      f"mesh ({tit / 1e6:.0f} M tet-iterations, {el:.1f} s), f32 oracle, "
+                      f"OpenMP over tets"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=64, help="hexes per axis")
+    ap.add_argument("--accumulate", default="atomic", choices=["atomic", "csr"])
+    ap.add_argument("--exec-path", default="graph", choices=["graph", "kernels"])
+    ap.add_argument("--cpu-substeps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--roofline-frames", type=int, default=1)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    from xpbd_gpu.scenes import cfg2_plain_large, make_solver
+
+    scene = cfg2_plain_large(n=args.n)
+    T, V = scene.mesh.n_tets, scene.mesh.n_verts
+    iters = scene.params["constraints_iter"]
+    subs = scene.params["substeps_num"]
+    print(f"[bench] workload cfg2: {V} verts, {T} tets, {subs} substeps x {iters} iterations "
+          f"per frame; accumulate={args.accumulate} exec={args.exec_path}", file=sys.stderr,
+          flush=True)
+    solver = make_solver(scene, accumulate_mode=args.accumulate, exec_path=args.exec_path)
+
+    for _ in range(args.warmup):
+        solver.solve()
+    solver.synchronize()
+
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        solver.solve()  # ends with a stream synchronization
+        lat.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - t0
+    solver.check_finite()
+    tet_iters = T * iters * subs * args.steps
+    value = tet_iters / elapsed
+    gbps = b_alg_iteration(T, V) * iters * subs * args.steps / elapsed / 1e9
+
+    # dominant kernel: HIP events around every projection launch (its stream)
+    solver.set_kernel_timing(True)
+    for _ in range(args.roofline_frames):
+        solver.solve()
+    ms, launches = solver.kernel_time()
+    solver.set_kernel_timing(False)
+    avg_s = ms / 1e3 / max(1, launches)
+    ach = b_alg_project(T, V) / avg_s / 1e9
+    roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "k_project", "avg_us": round(avg_s * 1e6, 2), "launches": launches,
+                "bytes_per_launch": b_alg_project(T, V)}
+
+    out = {
+        "metric": "tet-iterations/s & HBM GB/s at 1.57M tets; p99 step latency batched vs plain",
+        "value": value,
+        "unit": "tet-iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cfg2 single large mesh: {args.n}^3 jittered box, {T} tets, {V} verts, "
+                        f"{subs} substeps x {iters} iterations per step (one frame)",
+            "accumulate_mode": args.accumulate,
+            "exec_path": args.exec_path,
+        },
+        "hbm_gbps_algorithmic": gbps,
+        "b_alg_per_iteration": b_alg_iteration(T, V),
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                            "p99": float(np.percentile(lat, 99) * 1e3)},
+        "roofline": roofline,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

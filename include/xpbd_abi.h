@@ -1,0 +1,197 @@
+/*
+ * C ABI of the MI355X (gfx950) XPBD Neo-Hookean substep library, libxpbd_hip.so.
+ *
+ * The reference exposes this path as a Python class, not an FFI:
+ *   xpbd_gpu/neohookean_solver.py:15-587  XPBDNeoHookeanSolver
+ *   xpbd_gpu/xpbd_base.py:11-204          XPBDSolverBase
+ *   xpbd_gpu/sdf.py:59-95                 PlaneSdf, SphereSdf
+ * Each entry point below names the reference method it replaces.  The Python
+ * drop-in package (xpbd_gpu) binds these with ctypes.
+ *
+ * Conventions
+ *  - Every function returns 0 on success or a negative XPBD_E* code; the
+ *    message of the last error on the calling thread is read with
+ *    xpbd_last_error().
+ *  - Host buffers are copied in or out; the library owns all device memory.
+ *  - One solver handle is used from one host thread at a time.  All device
+ *    work of a handle is ordered on the handle's own HIP stream; the get_*
+ *    calls synchronize that stream, the step calls do not.
+ *  - Positions and velocities are float32, (n_verts, 3) row-major, in the
+ *    caller's vertex order (any internal reordering is hidden).
+ */
+#ifndef XPBD_ABI_H_
+#define XPBD_ABI_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPBD_OK 0
+#define XPBD_EINVAL -1   /* bad argument */
+#define XPBD_EHIP -2     /* HIP runtime error */
+#define XPBD_ENOMEM -3   /* device allocation failed */
+#define XPBD_ESTATE -4   /* call not valid in the current state */
+#define XPBD_ENUMERIC -5 /* NaN/Inf found in the state */
+
+#define XPBD_MAX_SDF 8
+
+/* accumulate_mode */
+#define XPBD_ACCUM_ATOMIC 0 /* float atomics into per-vertex accumulators */
+#define XPBD_ACCUM_CSR 1    /* deterministic vertex-side gather, fixed order */
+
+/* kind of an SDF collider (xpbd_gpu/sdf.py) */
+#define XPBD_SDF_PLANE 0  /* PlaneSdf(point, dir, fixed), sdf.py:59-74 */
+#define XPBD_SDF_SPHERE 1 /* SphereSdf(center, radius, fixed), sdf.py:77-95 */
+
+typedef struct xpbd_sdf {
+  int32_t kind;
+  int32_t fixed; /* pins vertices that start inside; always projects (sdf.py:24) */
+  double p[4];   /* plane: point xyz | sphere: center xyz, radius */
+  double dir[3]; /* plane: normal (normalized by the library, sdf.py:66) */
+} xpbd_sdf;
+
+/* Constructor arguments of XPBDNeoHookeanSolver (neohookean_solver.py:18-38)
+ * and XPBDSolverBase (xpbd_base.py:15-36), plus the MI355X execution options. */
+typedef struct xpbd_desc {
+  int32_t n_verts;
+  int32_t n_tets;
+  const float* rest_positions; /* n_verts*3, host */
+  const int32_t* tets;         /* n_tets*4, host, positively oriented */
+  double young_modulus;        /* default 2e5 */
+  double poisson_ratio;        /* default 0.495 */
+  double scale;                /* default 1 */
+  double offset[3];            /* default 0 */
+  double frame_dt;             /* default 1e-2 */
+  double dt;                   /* default 1e-2 */
+  int32_t substeps_num;        /* default 20 */
+  int32_t constraints_iter;    /* default 5 */
+  double hydrostatic_relaxation; /* default 1.2 (split mode) */
+  double deviatoric_relaxation;  /* default 1.2 (split mode) */
+  int32_t cheb_enable;         /* default 1 */
+  int32_t cheb_warmup;         /* default 5 */
+  double cheb_gamma;           /* default 0.6666 */
+  int32_t block_neohookean_enable; /* default 0 */
+  int32_t monitor_convergence; /* default 0: hydro/dev error per iteration */
+  double damping;              /* default 5 (xpbd_base.py:35) */
+  double gravity[3];           /* default (0,-9.8,0) (xpbd_base.py:33-34) */
+  double total_mass;           /* default 1 (xpbd_base.py:36) */
+  int32_t n_sdf;               /* >= 1 (xpbd_base.py:66-67) */
+  const xpbd_sdf* sdfs;
+  /* execution options (no reference counterpart) */
+  int32_t accumulate_mode;     /* XPBD_ACCUM_ATOMIC or XPBD_ACCUM_CSR */
+  int32_t reorder;             /* 1: Morton-reorder tets and vertices internally */
+  int32_t block_size;          /* threads per workgroup for tet kernels (64..1024) */
+  int32_t device;              /* HIP device ordinal */
+} xpbd_desc;
+
+typedef struct xpbd_solver xpbd_solver;
+
+/* Version of this ABI and the sizes of its structs as compiled, so that a
+ * binding can check its own struct layout. */
+#define XPBD_ABI_VERSION 1
+int xpbd_abi_info(int32_t* version, size_t* desc_size, size_t* sdf_size);
+
+/* Fills *d with the reference defaults (everything but the mesh). */
+int xpbd_desc_init(xpbd_desc* d);
+
+/* XPBDNeoHookeanSolver.__init__ (neohookean_solver.py:18-95): uploads the
+ * mesh, runs initialize (:97-107) and compute_lumped_mass (xpbd_base.py:133-137). */
+int xpbd_create(const xpbd_desc* desc, xpbd_solver** out);
+int xpbd_destroy(xpbd_solver* s);
+
+/* Refreshes scalar parameters (material, time step, Chebyshev, mode flags)
+ * from desc; the mesh fields of desc are ignored. */
+int xpbd_update_params(xpbd_solver* s, const xpbd_desc* desc);
+
+/* self.sdfs = ... (xpbd_base.py:61-70); does not re-pin vertices. */
+int xpbd_set_sdfs(xpbd_solver* s, const xpbd_sdf* sdfs, int32_t n);
+
+/* self.gravity[None] = g (xpbd_base.py:33-34) */
+int xpbd_set_gravity(xpbd_solver* s, const double g[3]);
+
+/* compute_lumped_mass with self.total_mass = total_mass (xpbd_base.py:89-137) */
+int xpbd_compute_lumped_mass(xpbd_solver* s, double total_mass);
+
+/* mesh.verts.x.to_numpy() / from_numpy() (xpbd_base.py:54, utils/renderer.py:49) */
+int xpbd_get_positions(xpbd_solver* s, float* xyz);
+int xpbd_set_positions(xpbd_solver* s, const float* xyz);
+/* Same, with a device pointer (n_verts*3 float32 in device memory, e.g. a
+ * torch-ROCm tensor); ordered on the handle's stream, synchronized on return. */
+int xpbd_get_positions_device(xpbd_solver* s, void* dev_xyz);
+int xpbd_set_positions_device(xpbd_solver* s, const void* dev_xyz);
+/* mesh.verts.v, mesh.verts.pred_x, mesh.verts.m / inv_m */
+int xpbd_get_velocities(xpbd_solver* s, float* xyz);
+int xpbd_set_velocities(xpbd_solver* s, const float* xyz);
+int xpbd_get_predicted(xpbd_solver* s, float* xyz);
+int xpbd_get_masses(xpbd_solver* s, float* m, float* inv_m);
+/* mesh.cells.lambda_D / lambda_H (neohookean_solver.py:87-92), caller tet order */
+int xpbd_get_multipliers(xpbd_solver* s, float* lambda_d, float* lambda_h);
+/* indices field (xpbd_base.py:57-58, 78-87): n_tets*12 uint32 face indices */
+int xpbd_get_visual_indices(xpbd_solver* s, uint32_t* out);
+
+/* apply_external_forces (xpbd_base.py:139-146) */
+int xpbd_apply_external_forces(xpbd_solver* s, double dt);
+/* solve_sdf_collision (xpbd_base.py:176-179) */
+int xpbd_solve_sdf_collision(xpbd_solver* s, double dt);
+/* solve_step (neohookean_solver.py:453-560) */
+int xpbd_solve_step(xpbd_solver* s, double dt);
+/* advance (xpbd_base.py:148-160) */
+int xpbd_advance(xpbd_solver* s, double dt, double damping);
+/* apply_external_forces + solve_step + advance (neohookean_solver.py:577-579) */
+int xpbd_substep(xpbd_solver* s, double sub_dt);
+/* solve() x n_frames (neohookean_solver.py:562-587) */
+int xpbd_frame(xpbd_solver* s, int32_t n_frames);
+
+/* Waits for the handle's stream; reports XPBD_EHIP on an asynchronous error. */
+int xpbd_synchronize(xpbd_solver* s);
+/* The handle's hipStream_t, for callers that order their own work after it. */
+int xpbd_get_stream(xpbd_solver* s, void** stream);
+
+/* Per-iteration log.  When enabled, each substep's rows are copied to the
+ * host at the end of the substep (one synchronization per substep).
+ * Row (8 doubles): frame, substep, iter, r_hat, rho_hat, omega, hydro_error,
+ * dev_error.  r_hat is not logged by the reference (neohookean_solver.py:496). */
+int xpbd_set_logging(xpbd_solver* s, int32_t enabled);
+int xpbd_log_size(xpbd_solver* s, int64_t* rows);
+int xpbd_get_log(xpbd_solver* s, double* rows, int64_t max_rows);
+int xpbd_clear_log(xpbd_solver* s);
+
+/* Solver counters: current_frame, current_substep, time, rho_hat
+ * (neohookean_solver.py:55-56, 65; xpbd_base.py:27). */
+int xpbd_get_state(xpbd_solver* s, int32_t* frame, int32_t* substep, double* time,
+                   double* rho_hat);
+/* rho_hat = NaN keeps the device value. */
+int xpbd_set_state(xpbd_solver* s, int32_t frame, double time, double rho_hat);
+
+/* Size of the mesh as stored on the device (after padding). */
+int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets);
+
+/* Execution-path selection (no reference counterpart).
+ *  XPBD_PATH_KERNELS:    one launch per kernel per iteration.
+ *  XPBD_PATH_GRAPH:      the same kernels, each substep captured in a hipGraph. */
+#define XPBD_PATH_KERNELS 0
+#define XPBD_PATH_GRAPH 1
+int xpbd_set_exec_path(xpbd_solver* s, int32_t path);
+
+/* Kernel timing (no reference counterpart): when enabled, the library brackets
+ * every launch of the tet projection kernel with HIP events on its stream;
+ * xpbd_get_kernel_time returns the summed milliseconds and launch count since
+ * the last reset (synchronizes). */
+int xpbd_set_kernel_timing(xpbd_solver* s, int32_t enabled);
+int xpbd_get_kernel_time(xpbd_solver* s, double* total_ms, int64_t* launches);
+
+/* Reports XPBD_ENUMERIC when any vertex position (x or pred_x) is NaN or
+ * infinite; the reference never checks (xpbd_base.py:139-160). */
+int xpbd_check_finite(xpbd_solver* s);
+
+/* Copies the last error message of the calling thread into buf. */
+int xpbd_last_error(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XPBD_ABI_H_ */

@@ -1,0 +1,975 @@
+// CPU restatement of the reference XPBD Neo-Hookean solver (TEST INFRASTRUCTURE).
+//
+// This file is the correctness oracle and the timed CPU baseline.  It is never
+// part of the product path: only tests, the smoke check and the benchmark's
+// CPU-baseline leg load it.  It restates, kernel by kernel and in the same
+// order, the reference solver written in Taichi:
+//
+//   xpbd_gpu/neohookean_solver.py   (RaymondMcGuire/rdc-xpbd-taichi)
+//   xpbd_gpu/xpbd_base.py
+//   xpbd_gpu/sdf.py
+//   xpbd_gpu/constants.py
+//
+// Each function names the reference lines it follows.  The template parameter
+// R is the arithmetic type: float restates the reference's f32 fields, double
+// is the FP64 run used for the error-vs-FP64 figures.
+//
+// Scatter semantics.  The reference accumulates per-vertex corrections with
+// float atomics (neohookean_solver.py:443-451), whose order is unspecified.
+// This oracle fixes one order: ascending tet index.  Per-tet corner values are
+// computed (optionally with OpenMP threads) into a buffer and summed per
+// vertex through a vertex->(tet,corner) adjacency list sorted by tet, which is
+// bit-identical to a sequential scatter loop over tets, for any thread count.
+// Scalar sums (residual, mass, errors) are likewise summed sequentially in tet
+// order.  Host-side scalars that the reference keeps in Python doubles (rho,
+// omega, time) are doubles here for both R.
+//
+// Built with -ffp-contract=off so that no fused multiply-add changes the op
+// order written below.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kEpsilon = 1e-12;  // xpbd_gpu/constants.py:3
+
+template <typename R>
+struct M3 {  // row-major 3x3
+  R m[3][3];
+};
+
+template <typename R>
+inline void cross3(const R a[3], const R b[3], R out[3]) {
+  // ti.Vector.cross: (a1 b2 - a2 b1, a2 b0 - a0 b2, a0 b1 - a1 b0)
+  out[0] = a[1] * b[2] - a[2] * b[1];
+  out[1] = a[2] * b[0] - a[0] * b[2];
+  out[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+template <typename R>
+inline R dot3(const R a[3], const R b[3]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+// 3x3 determinant by cofactor expansion along the first column (the form
+// Taichi's Matrix.determinant uses for n == 3).
+template <typename R>
+inline R det3(const M3<R>& a) {
+  const auto& m = a.m;
+  return m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+         m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+         m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+}
+
+// 3x3 inverse as adjugate / determinant (Matrix.inverse for n == 3).
+template <typename R>
+inline M3<R> inv3(const M3<R>& a) {
+  const auto& m = a.m;
+  const R inv_det = R(1) / det3(a);
+  M3<R> r;
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      r.m[j][i] = inv_det * (m[i1][j1] * m[i2][j2] - m[i2][j1] * m[i1][j2]);
+    }
+  }
+  return r;
+}
+
+// A @ B
+template <typename R>
+inline M3<R> mul(const M3<R>& a, const M3<R>& b) {
+  M3<R> r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+  return r;
+}
+
+// A @ B^T
+template <typename R>
+inline M3<R> mul_bt(const M3<R>& a, const M3<R>& b) {
+  M3<R> r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      r.m[i][j] = a.m[i][0] * b.m[j][0] + a.m[i][1] * b.m[j][1] + a.m[i][2] * b.m[j][2];
+  return r;
+}
+
+// Matrix.norm_sqr(): sum of squared entries, row-major.
+template <typename R>
+inline R norm_sqr(const M3<R>& a) {
+  R s = R(0);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) s += a.m[i][j] * a.m[i][j];
+  return s;
+}
+
+// F = Ds @ inv_Dm with Ds = cols(p1-p0, p2-p0, p3-p0)
+// (neohookean_solver.py:371-373).
+template <typename R>
+inline M3<R> deformation_gradient(const R* p0, const R* p1, const R* p2, const R* p3,
+                                  const M3<R>& inv_dm) {
+  M3<R> ds;
+  for (int r = 0; r < 3; ++r) {
+    ds.m[r][0] = p1[r] - p0[r];
+    ds.m[r][1] = p2[r] - p0[r];
+    ds.m[r][2] = p3[r] - p0[r];
+  }
+  return mul(ds, inv_dm);
+}
+
+// Gradients of C w.r.t. the 4 vertices from dC/dF: columns of dC/dF @ inv_Dm^T
+// for vertices 1..3, and g0 = -(g1+g2+g3) (neohookean_solver.py:382-404).
+template <typename R>
+inline void vertex_grads(const M3<R>& dcdf, const M3<R>& inv_dm, R g[4][3]) {
+  const M3<R> g123 = mul_bt(dcdf, inv_dm);
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) g[c + 1][r] = g123.m[r][c];
+  for (int r = 0; r < 3; ++r) g[0][r] = -(g[1][r] + g[2][r] + g[3][r]);
+}
+
+// dC_H/dF = cols(F1 x F2, F2 x F0, F0 x F1) (neohookean_solver.py:391-398).
+template <typename R>
+inline M3<R> ddet_dF(const M3<R>& f) {
+  R c0[3], c1[3], c2[3], x[3];
+  for (int r = 0; r < 3; ++r) {
+    c0[r] = f.m[r][0];
+    c1[r] = f.m[r][1];
+    c2[r] = f.m[r][2];
+  }
+  M3<R> g;
+  cross3(c1, c2, x);
+  for (int r = 0; r < 3; ++r) g.m[r][0] = x[r];
+  cross3(c2, c0, x);
+  for (int r = 0; r < 3; ++r) g.m[r][1] = x[r];
+  cross3(c0, c1, x);
+  for (int r = 0; r < 3; ++r) g.m[r][2] = x[r];
+  return g;
+}
+
+// solve2x2_pivot_lu (neohookean_solver.py:331-356).  Returns ok.
+template <typename R>
+inline int solve2x2_pivot_lu(R a00, R a01, R a10, R a11, R b0, R b1, R eps, R* x0, R* x1) {
+  int ok = 1;
+  if (std::fabs(a00) < std::fabs(a10)) {
+    std::swap(a00, a10);
+    std::swap(a01, a11);
+    std::swap(b0, b1);
+  }
+  if (std::fabs(a00) < eps) ok = 0;
+  *x0 = R(0);
+  *x1 = R(0);
+  if (ok == 1) {
+    const R m = a10 / a00;
+    const R a11_ = a11 - m * a01;
+    const R b1_ = b1 - m * b0;
+    if (std::fabs(a11_) < eps) {
+      ok = 0;
+    } else {
+      *x1 = b1_ / a11_;
+      *x0 = (b0 - a01 * *x1) / a00;
+    }
+  }
+  return ok;
+}
+
+struct SdfDesc {
+  int kind;        // 0 plane, 1 sphere
+  int fixed;
+  double p[4];     // plane: point xyz | sphere: center xyz, radius
+  double dir[3];   // plane: unit normal (normalized in double at construction, sdf.py:66)
+};
+
+template <typename R>
+struct Sdf {
+  int kind, fixed;
+  R p[3], r, d[3];
+  explicit Sdf(const SdfDesc& s) : kind(s.kind), fixed(s.fixed) {
+    for (int i = 0; i < 3; ++i) {
+      p[i] = R(s.p[i]);
+      d[i] = R(s.dir[i]);
+    }
+    r = R(s.p[3]);
+  }
+  // PlaneSdf.dist (sdf.py:69-70) / SphereSdf.dist (sdf.py:87-88)
+  R dist(const R* x) const {
+    R q[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+    if (kind == 0) return dot3(q, d);
+    return std::sqrt(dot3(q, q)) - r;
+  }
+  // PlaneSdf.normal (sdf.py:72-74) / SphereSdf.normal = normalized(EPSILON) (sdf.py:90-92)
+  void normal(const R* x, R* n) const {
+    if (kind == 0) {
+      for (int i = 0; i < 3; ++i) n[i] = d[i];
+      return;
+    }
+    R q[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
+    const R inv_len = R(1) / (std::sqrt(dot3(q, q)) + R(kEpsilon));
+    for (int i = 0; i < 3; ++i) n[i] = inv_len * q[i];
+  }
+  // SdfModel.check (sdf.py:13-27): returns inside, fills n.  The obstacle's
+  // own velocity is always zero in the reference (never written).
+  bool check(const R* x, const R* v, R* n) const {
+    const R phi = dist(x);
+    bool inside = false;
+    n[0] = n[1] = n[2] = R(0);
+    if (phi < R(0)) {
+      normal(x, n);
+      const R dv[3] = {R(0) - v[0], R(0) - v[1], R(0) - v[2]};
+      const R dotnv = dot3(n, dv);
+      if (dotnv > R(0) || fixed) inside = true;
+    }
+    return inside;
+  }
+};
+
+struct Params {
+  double young, poisson, scale, offset[3];
+  double frame_dt, dt;
+  int substeps, iters;
+  double hydro_relax, dev_relax;
+  int cheb_enable, cheb_warmup;
+  double cheb_gamma;
+  int block_enable, monitor;
+  double damping, gravity[3], total_mass;
+};
+
+struct IterLog {
+  int frame, substep, iter;
+  double r_hat, rho_hat, omega, hydro_error, dev_error;
+};
+
+template <typename R>
+class Solver {
+ public:
+  Solver(int nv, int nt, const float* rest, const int* tets, const Params& prm,
+         const SdfDesc* sdfs, int n_sdf, int threads)
+      : nv_(nv), nt_(nt), prm_(prm), threads_(threads) {
+    tets_.assign(tets, tets + 4 * size_t(nt));
+    for (int i = 0; i < n_sdf; ++i) sdfs_.emplace_back(sdfs[i]);
+    x_.resize(3 * size_t(nv));
+    for (size_t i = 0; i < 3 * size_t(nv); ++i) x_[i] = R(rest[i]);
+    pred_.assign(3 * size_t(nv), R(0));
+    v_.assign(3 * size_t(nv), R(0));
+    m_.assign(nv, R(0));
+    inv_m_.assign(nv, R(0));
+    dx_.assign(3 * size_t(nv), R(0));
+    cons_.assign(nv, 0u);
+    prev_.assign(3 * size_t(nv), R(0));
+    curr_.assign(3 * size_t(nv), R(0));
+    V0_.assign(nt, R(0));
+    inv_dm_.resize(nt);
+    lam_h_.assign(nt, R(0));
+    lam_d_.assign(nt, R(0));
+    corner_a_.assign(12 * size_t(nt), R(0));
+    corner_b_.assign(12 * size_t(nt), R(0));
+    corner_ok_.assign(2 * size_t(nt), 0);
+    per_tet_.assign(nt, R(0));
+    per_tet2_.assign(nt, R(0));
+    per_tet3_.assign(nt, R(0));
+    build_adjacency();
+    set_lame();
+    gravity_[0] = R(prm_.gravity[0]);
+    gravity_[1] = R(prm_.gravity[1]);
+    gravity_[2] = R(prm_.gravity[2]);
+    initialize(R(prm_.scale), prm_.offset);
+    total_mass_ = prm_.total_mass;
+    compute_lumped_mass();
+  }
+
+  // ---- state access -------------------------------------------------------
+  void get_x(double* out) const {
+    for (size_t i = 0; i < x_.size(); ++i) out[i] = double(x_[i]);
+  }
+  void set_x(const double* in) {
+    for (size_t i = 0; i < x_.size(); ++i) x_[i] = R(in[i]);
+  }
+  void get_v(double* out) const {
+    for (size_t i = 0; i < v_.size(); ++i) out[i] = double(v_[i]);
+  }
+  void set_v(const double* in) {
+    for (size_t i = 0; i < v_.size(); ++i) v_[i] = R(in[i]);
+  }
+  void get_pred(double* out) const {
+    for (size_t i = 0; i < pred_.size(); ++i) out[i] = double(pred_[i]);
+  }
+  void get_mass(double* m, double* inv_m) const {
+    for (int i = 0; i < nv_; ++i) {
+      m[i] = double(m_[i]);
+      inv_m[i] = double(inv_m_[i]);
+    }
+  }
+  void get_tet_state(double* v0, double* inv_dm, double* lam_d, double* lam_h) const {
+    for (int t = 0; t < nt_; ++t) {
+      if (v0) v0[t] = double(V0_[t]);
+      if (inv_dm)
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) inv_dm[9 * size_t(t) + 3 * i + j] = double(inv_dm_[t].m[i][j]);
+      if (lam_d) lam_d[t] = double(lam_d_[t]);
+      if (lam_h) lam_h[t] = double(lam_h_[t]);
+    }
+  }
+  void set_gravity(const double* g) {
+    for (int i = 0; i < 3; ++i) gravity_[i] = R(g[i]);
+  }
+  void set_total_mass(double tm) { total_mass_ = tm; }
+  double rho_hat() const { return rho_hat_; }
+  double time() const { return time_; }
+  int frame() const { return frame_; }
+  const std::vector<IterLog>& log() const { return log_; }
+  void clear_log() { log_.clear(); }
+
+  // ---- setup kernels --------------------------------------------------------
+
+  // calc_first_lame / calc_second_lame (neohookean_solver.py:7-12), evaluated
+  // in Python doubles and frozen into the kernels as f32 constants.
+  void set_lame() {
+    const double E = prm_.young, nu = prm_.poisson;
+    const double mu = E / (2.0 * (1.0 + nu));
+    const double la = 2.0 * nu / (1.0 - 2.0 * nu) * mu;
+    lame1_ = R(la);
+    lame2_ = R(mu);
+  }
+
+  // initialize (neohookean_solver.py:97-107) with calc_tet_volume
+  // (xpbd_base.py:6-8).
+  void initialize(R scale, const double* offset) {
+    const R off[3] = {R(offset[0]), R(offset[1]), R(offset[2])};
+    for (int v = 0; v < nv_; ++v)
+      for (int k = 0; k < 3; ++k) x_[3 * v + k] = x_[3 * v + k] * scale + off[k];
+    for (int t = 0; t < nt_; ++t) {
+      const R* p[4];
+      for (int k = 0; k < 4; ++k) p[k] = &x_[3 * size_t(tets_[4 * size_t(t) + k])];
+      V0_[t] = tet_volume(p[0], p[1], p[2], p[3]);
+      M3<R> dm;
+      for (int r = 0; r < 3; ++r) {
+        dm.m[r][0] = p[1][r] - p[0][r];
+        dm.m[r][1] = p[2][r] - p[0][r];
+        dm.m[r][2] = p[3][r] - p[0][r];
+      }
+      inv_dm_[t] = inv3(dm);
+    }
+  }
+
+  static R tet_volume(const R* p0, const R* p1, const R* p2, const R* p3) {
+    const R a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const R b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const R c[3] = {p3[0] - p0[0], p3[1] - p0[1], p3[2] - p0[2]};
+    R ab[3];
+    cross3(a, b, ab);
+    return dot3(ab, c) / R(6);
+  }
+
+  // _compute_lumped_mass_kernel + _apply_fixed_vertices_for_sdf
+  // (xpbd_base.py:89-137).
+  void compute_lumped_mass() {
+    for (int t = 0; t < nt_; ++t) {
+      const R* p[4];
+      for (int k = 0; k < 4; ++k) p[k] = &x_[3 * size_t(tets_[4 * size_t(t) + k])];
+      per_tet_[t] = tet_volume(p[0], p[1], p[2], p[3]);
+    }
+    R total_volume = R(0);
+    for (int v = 0; v < nv_; ++v) m_[v] = R(0);
+    for (int t = 0; t < nt_; ++t) {
+      const R quarter = per_tet_[t] * R(0.25);
+      for (int k = 0; k < 4; ++k) m_[tets_[4 * size_t(t) + k]] += quarter;
+      total_volume += per_tet_[t];
+    }
+    const R mass_scale = R(total_mass_) / total_volume;
+    for (int v = 0; v < nv_; ++v) {
+      m_[v] *= mass_scale;
+      inv_m_[v] = m_[v] > R(0) ? R(1) / m_[v] : R(0);
+    }
+    for (const auto& s : sdfs_) {
+      for (int v = 0; v < nv_; ++v) {
+        if (inv_m_[v] > R(0)) {
+          R n[3];
+          const bool inside = s.check(&x_[3 * size_t(v)], &v_[3 * size_t(v)], n);
+          if (s.fixed && inside) inv_m_[v] = R(0);
+        }
+      }
+    }
+  }
+
+  // ---- per-substep kernels -------------------------------------------------
+
+  // apply_external_forces (xpbd_base.py:139-146)
+  void apply_external_forces(R dt) {
+#pragma omp parallel for num_threads(threads_) schedule(static)
+    for (int v = 0; v < nv_; ++v) {
+      if (inv_m_[v] > R(0))
+        for (int k = 0; k < 3; ++k) v_[3 * v + k] += gravity_[k] * dt;
+      for (int k = 0; k < 3; ++k) pred_[3 * v + k] = x_[3 * v + k] + v_[3 * v + k] * dt;
+    }
+  }
+
+  // _solve_sdf_collision_one / solve_sdf_collision (xpbd_base.py:162-179)
+  void solve_sdf_collision() {
+    for (const auto& s : sdfs_) {
+#pragma omp parallel for num_threads(threads_) schedule(static)
+      for (int v = 0; v < nv_; ++v) {
+        if (inv_m_[v] > R(0)) {
+          R n[3];
+          R* x = &pred_[3 * size_t(v)];
+          if (s.check(x, &v_[3 * size_t(v)], n)) {
+            const R depth = -s.dist(x);
+            if (depth > R(0))
+              for (int k = 0; k < 3; ++k) x[k] += n[k] * depth;
+          }
+        }
+      }
+    }
+  }
+
+  // advance (xpbd_base.py:148-160).  ti.exp of an f32 argument.
+  void advance(R dt, R damping) {
+    const R decay = std::exp(-dt * damping);
+#pragma omp parallel for num_threads(threads_) schedule(static)
+    for (int v = 0; v < nv_; ++v) {
+      if (inv_m_[v] <= R(0)) {
+        for (int k = 0; k < 3; ++k) pred_[3 * v + k] = x_[3 * v + k];
+      } else {
+        for (int k = 0; k < 3; ++k) {
+          const R raw_v = (pred_[3 * v + k] - x_[3 * v + k]) / dt;
+          v_[3 * v + k] = raw_v * decay;
+          x_[3 * v + k] = pred_[3 * v + k];
+        }
+      }
+    }
+  }
+
+  // ---- Jacobi iteration kernels ----------------------------------------------
+
+  // Per-tet block projection of solve_neohookean_block
+  // (neohookean_solver.py:358-451).  Writes the 4 corner corrections into
+  // `corner` (12 values) and updates the multipliers.
+  void project_block_tet(int t, R dt, R* corner) {
+    const int* tv = &tets_[4 * size_t(t)];
+    const R* p[4];
+    R w[4];
+    for (int k = 0; k < 4; ++k) {
+      p[k] = &pred_[3 * size_t(tv[k])];
+      w[k] = inv_m_[tv[k]];
+    }
+    const M3<R>& idm = inv_dm_[t];
+    const M3<R> F = deformation_gradient(p[0], p[1], p[2], p[3], idm);
+    const R C_D = norm_sqr(F) - R(3);
+    const R C_H = det3(F) - R(1);
+    M3<R> gfd;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * R(2);
+    R gD[4][3], gH[4][3];
+    vertex_grads(gfd, idm, gD);
+    vertex_grads(ddet_dF(F), idm, gH);
+    R A11 = w[0] * dot3(gD[0], gD[0]) + w[1] * dot3(gD[1], gD[1]) +
+            w[2] * dot3(gD[2], gD[2]) + w[3] * dot3(gD[3], gD[3]);
+    R A22 = w[0] * dot3(gH[0], gH[0]) + w[1] * dot3(gH[1], gH[1]) +
+            w[2] * dot3(gH[2], gH[2]) + w[3] * dot3(gH[3], gH[3]);
+    const R A12 = w[0] * dot3(gD[0], gH[0]) + w[1] * dot3(gD[1], gH[1]) +
+                  w[2] * dot3(gD[2], gH[2]) + w[3] * dot3(gD[3], gH[3]);
+    const R alpha_D = R(1) / (lame2_ * V0_[t]);
+    const R alpha_H = R(1) / (lame1_ * V0_[t]);
+    const R aD = alpha_D / (dt * dt);
+    const R aH = alpha_H / (dt * dt);
+    A11 += aD;
+    A22 += aH;
+    const R b1 = -(C_D + aD * lam_d_[t]);
+    const R b2 = -(C_H + aH * lam_h_[t]);
+    const R eps = R(kEpsilon);
+    R dlD, dlH;
+    const int ok = solve2x2_pivot_lu(A11, A12, A12, A22, b1, b2, eps, &dlD, &dlH);
+    if (ok == 0) {  // neohookean_solver.py:425-432
+      dlD = R(0);
+      dlH = R(0);
+      if (std::fabs(A11) > eps) dlD = b1 / A11;
+      if (std::fabs(A22) > eps) dlH = b2 / A22;
+    }
+    lam_d_[t] += dlD;
+    lam_h_[t] += dlH;
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) corner[3 * k + r] = w[k] * (gD[k][r] * dlD + gH[k][r] * dlH);
+  }
+
+  // One scalar constraint of split mode: solve_hydrostatic
+  // (neohookean_solver.py:196-254) when hydro, solve_deviatoric (:144-194)
+  // otherwise.  Returns whether w_sum > EPSILON (the cons_num increment).
+  int project_split_tet(int t, R dt, bool hydro, R* corner) {
+    const int* tv = &tets_[4 * size_t(t)];
+    const R* p[4];
+    R w[4];
+    for (int k = 0; k < 4; ++k) {
+      p[k] = &pred_[3 * size_t(tv[k])];
+      w[k] = inv_m_[tv[k]];
+    }
+    const M3<R>& idm = inv_dm_[t];
+    const M3<R> F = deformation_gradient(p[0], p[1], p[2], p[3], idm);
+    R C;
+    M3<R> gf;
+    if (hydro) {
+      C = det3(F) - R(1);
+      gf = ddet_dF(F);
+    } else {
+      C = norm_sqr(F) - R(3);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) gf.m[i][j] = R(2) * F.m[i][j];
+    }
+    R g[4][3];
+    vertex_grads(gf, idm, g);
+    R w_sum = w[0] * dot3(g[0], g[0]);
+    w_sum += w[1] * dot3(g[1], g[1]);
+    w_sum += w[2] * dot3(g[2], g[2]);
+    w_sum += w[3] * dot3(g[3], g[3]);
+    if (!(w_sum > R(kEpsilon))) return 0;
+    const R lame = hydro ? lame1_ : lame2_;
+    const R alpha = R(1) / (lame * V0_[t]);
+    const R alpha_tilde = alpha / (dt * dt);
+    R& lam = hydro ? lam_h_[t] : lam_d_[t];
+    R d_lambda = -(C + alpha_tilde * lam) / (w_sum + alpha_tilde);
+    d_lambda *= R(hydro ? prm_.hydro_relax : prm_.dev_relax);
+    lam += d_lambda;
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) corner[3 * k + r] = d_lambda * w[k] * g[k][r];
+    return 1;
+  }
+
+  // pre_solve + projection + post_solve (neohookean_solver.py:109-119, 482-490).
+  void jacobi_iteration(R dt) {
+    if (prm_.block_enable) {
+#pragma omp parallel for num_threads(threads_) schedule(static)
+      for (int t = 0; t < nt_; ++t) project_block_tet(t, dt, &corner_a_[12 * size_t(t)]);
+    } else {
+      // solve_hydrostatic runs over all tets before solve_deviatoric.
+#pragma omp parallel for num_threads(threads_) schedule(static)
+      for (int t = 0; t < nt_; ++t) {
+        corner_ok_[2 * size_t(t)] = project_split_tet(t, dt, true, &corner_a_[12 * size_t(t)]);
+        corner_ok_[2 * size_t(t) + 1] = project_split_tet(t, dt, false, &corner_b_[12 * size_t(t)]);
+      }
+    }
+    // delta_x / cons_num accumulated in ascending tet order, then post_solve.
+#pragma omp parallel for num_threads(threads_) schedule(static)
+    for (int v = 0; v < nv_; ++v) {
+      R d[3] = {R(0), R(0), R(0)};
+      uint32_t cnt = 0;
+      const int b = adj_off_[v], e = adj_off_[v + 1];
+      if (prm_.block_enable) {
+        for (int a = b; a < e; ++a) {
+          const R* c = &corner_a_[3 * size_t(adj_[a])];
+          d[0] += c[0];
+          d[1] += c[1];
+          d[2] += c[2];
+          ++cnt;
+        }
+      } else {
+        for (int pass = 0; pass < 2; ++pass) {
+          const std::vector<R>& buf = pass == 0 ? corner_a_ : corner_b_;
+          for (int a = b; a < e; ++a) {
+            const int slot = adj_[a];
+            if (!corner_ok_[2 * size_t(slot / 4) + pass]) continue;
+            const R* c = &buf[3 * size_t(slot)];
+            d[0] += c[0];
+            d[1] += c[1];
+            d[2] += c[2];
+            ++cnt;
+          }
+        }
+      }
+      for (int k = 0; k < 3; ++k) dx_[3 * v + k] = d[k];
+      cons_[v] = cnt;
+      if (cnt > 0)
+        for (int k = 0; k < 3; ++k) pred_[3 * v + k] += d[k] / R(cnt);
+    }
+  }
+
+  // compute_xpbd_residual (neohookean_solver.py:298-329), plus the monitor
+  // kernels compute_hydrostatic_error / compute_deviatoric_error (:256-296)
+  // evaluated on the same positions.  Returns sum r (in R) and fills the sums
+  // of |C_H| and |C_D|.
+  R residual(R dt, R* hydro_sum, R* dev_sum) {
+#pragma omp parallel for num_threads(threads_) schedule(static)
+    for (int t = 0; t < nt_; ++t) {
+      const int* tv = &tets_[4 * size_t(t)];
+      const M3<R> F = deformation_gradient(&pred_[3 * size_t(tv[0])], &pred_[3 * size_t(tv[1])],
+                                           &pred_[3 * size_t(tv[2])], &pred_[3 * size_t(tv[3])],
+                                           inv_dm_[t]);
+      const R C_D = norm_sqr(F) - R(3);
+      const R C_H = det3(F) - R(1);
+      const R alpha_D = R(1) / (lame2_ * V0_[t]);
+      const R alpha_H = R(1) / (lame1_ * V0_[t]);
+      const R aD = alpha_D / (dt * dt);
+      const R aH = alpha_H / (dt * dt);
+      const R rD = C_D + aD * lam_d_[t];
+      const R rH = C_H + aH * lam_h_[t];
+      per_tet_[t] = std::sqrt(rD * rD + rH * rH);
+      per_tet2_[t] = C_H;
+      per_tet3_[t] = C_D;
+    }
+    R s = R(0), sh = R(0), sd = R(0);
+    for (int t = 0; t < nt_; ++t) {
+      s += per_tet_[t];
+      sh += std::fabs(per_tet2_[t]);
+      sd += std::fabs(per_tet3_[t]);
+    }
+    *hydro_sum = sh;
+    *dev_sum = sd;
+    return s;
+  }
+
+  // solve_step (neohookean_solver.py:453-560)
+  void solve_step(double sub_dt) {
+    const R dt = R(sub_dt);
+    std::fill(lam_h_.begin(), lam_h_.end(), R(0));
+    std::fill(lam_d_.begin(), lam_d_.end(), R(0));
+    bool have_prev_r = false;
+    double prev_r_hat = 0.0;
+    solve_sdf_collision();
+    const bool use_cheb = prm_.cheb_enable != 0;
+    double omega = 1.0;
+    if (use_cheb) prev_ = pred_;  // _save_curr_to_prev
+    const R gamma = R(prm_.cheb_gamma);
+    for (int it = 0; it < prm_.iters; ++it) {
+      if (use_cheb) curr_ = pred_;  // _save_curr_to_savebuf
+      jacobi_iteration(dt);
+      double r_hat = 0.0;
+      R hs = R(0), ds = R(0);
+      const bool need_res = use_cheb || prm_.monitor;
+      R rs = R(0);
+      if (need_res) rs = residual(dt, &hs, &ds);
+      if (use_cheb) {
+        // neohookean_solver.py:494-504
+        const int cnt = nt_;
+        r_hat = cnt > 0 ? double(rs) / double(cnt) : 0.0;
+        if (have_prev_r && prev_r_hat > kEpsilon) {
+          double rho_inst = r_hat / (prev_r_hat + kEpsilon);
+          rho_inst = std::max(kRhoMin, std::min(kRhoMax, rho_inst));
+          rho_hat_ = (1.0 - kRhoBeta) * rho_hat_ + kRhoBeta * rho_inst;
+        }
+        prev_r_hat = r_hat;
+        have_prev_r = true;
+        // neohookean_solver.py:537-546
+        const double rho = rho_hat_;
+        const double rho2 = rho * rho;
+        if (it < prm_.cheb_warmup)
+          omega = 1.0;
+        else if (it == prm_.cheb_warmup)
+          omega = 2.0 / (2.0 - rho2);
+        else
+          omega = 4.0 / (4.0 - rho2 * omega);
+        chebyshev_update(R(omega), gamma);
+        prev_ = curr_;  // _rotate_prev_with_save
+      }
+      IterLog rec;
+      rec.frame = frame_;
+      rec.substep = substep_;
+      rec.iter = it;
+      rec.r_hat = r_hat;
+      rec.rho_hat = rho_hat_;
+      rec.omega = use_cheb ? omega : 1.0;
+      const double nc = nt_ > 0 ? double(nt_) : 1.0;
+      rec.hydro_error = prm_.monitor && nt_ > 0 ? double(hs) / nc : 0.0;
+      rec.dev_error = prm_.monitor && nt_ > 0 ? double(ds) / nc : 0.0;
+      if (log_enabled_) log_.push_back(rec);
+    }
+  }
+
+  // _chebyshev_update (neohookean_solver.py:131-137)
+  void chebyshev_update(R omega, R gamma) {
+#pragma omp parallel for num_threads(threads_) schedule(static)
+    for (size_t i = 0; i < pred_.size(); ++i) {
+      const R x_new = gamma * (pred_[i] - curr_[i]) + curr_[i];
+      pred_[i] = prev_[i] + omega * (x_new - prev_[i]);
+    }
+  }
+
+  // solve (neohookean_solver.py:562-587)
+  void solve_frame() {
+    double frame_time_left = prm_.frame_dt;
+    substep_ = 0;
+    while (frame_time_left > 0.0) {
+      const double dt0 = std::min(prm_.dt, frame_time_left);
+      frame_time_left -= dt0;
+      const double sub_dt = dt0 / prm_.substeps;
+      for (int s = 0; s < prm_.substeps; ++s) {
+        substep_ = s;
+        substep(sub_dt);
+      }
+      time_ += dt0;
+    }
+    frame_ += 1;
+  }
+
+  // apply_external_forces -> solve_step -> advance (neohookean_solver.py:577-579)
+  void substep(double sub_dt) {
+    apply_external_forces(R(sub_dt));
+    solve_step(sub_dt);
+    advance(R(sub_dt), R(prm_.damping));
+  }
+
+  bool log_enabled_ = true;
+
+ private:
+  static constexpr double kRhoMin = 0.00001, kRhoMax = 0.99999, kRhoBeta = 0.2;
+
+  void build_adjacency() {
+    adj_off_.assign(nv_ + 1, 0);
+    for (size_t i = 0; i < tets_.size(); ++i) adj_off_[tets_[i] + 1]++;
+    for (int v = 0; v < nv_; ++v) adj_off_[v + 1] += adj_off_[v];
+    adj_.resize(tets_.size());
+    std::vector<int> fill(adj_off_.begin(), adj_off_.end() - 1);
+    for (int t = 0; t < nt_; ++t)  // ascending t -> each list sorted by tet
+      for (int k = 0; k < 4; ++k) adj_[fill[tets_[4 * size_t(t) + k]]++] = 4 * t + k;
+  }
+
+  int nv_, nt_;
+  Params prm_;
+  int threads_;
+  std::vector<int> tets_;
+  std::vector<Sdf<R>> sdfs_;
+  std::vector<R> x_, pred_, v_, m_, inv_m_, dx_, prev_, curr_;
+  std::vector<uint32_t> cons_;
+  std::vector<R> V0_, lam_h_, lam_d_;
+  std::vector<M3<R>> inv_dm_;
+  std::vector<R> corner_a_, corner_b_, per_tet_, per_tet2_, per_tet3_;
+  std::vector<int> corner_ok_;
+  std::vector<int> adj_off_, adj_;
+  R lame1_, lame2_;
+  R gravity_[3];
+  double total_mass_ = 1.0;
+  double rho_hat_ = 0.999;  // neohookean_solver.py:65, persists across substeps
+  double time_ = 0.0;
+  int frame_ = 0, substep_ = 0;
+  std::vector<IterLog> log_;
+};
+
+struct Handle {
+  int precision;  // 32 or 64
+  Solver<float>* s32 = nullptr;
+  Solver<double>* s64 = nullptr;
+};
+
+template <typename F>
+auto dispatch(Handle* h, F&& f) {
+  if (h->precision == 32) return f(*h->s32);
+  return f(*h->s64);
+}
+
+Params params_from(const double* p) {
+  Params q;
+  q.young = p[0];
+  q.poisson = p[1];
+  q.scale = p[2];
+  q.offset[0] = p[3];
+  q.offset[1] = p[4];
+  q.offset[2] = p[5];
+  q.frame_dt = p[6];
+  q.dt = p[7];
+  q.substeps = int(p[8]);
+  q.iters = int(p[9]);
+  q.hydro_relax = p[10];
+  q.dev_relax = p[11];
+  q.cheb_enable = int(p[12]);
+  q.cheb_warmup = int(p[13]);
+  q.cheb_gamma = p[14];
+  q.block_enable = int(p[15]);
+  q.monitor = int(p[16]);
+  q.damping = p[17];
+  q.gravity[0] = p[18];
+  q.gravity[1] = p[19];
+  q.gravity[2] = p[20];
+  q.total_mass = p[21];
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+// params: 22 doubles, in the order read by params_from().
+// sdf_kind[n] (0 plane, 1 sphere), sdf_fixed[n], sdf_p[4n], sdf_dir[3n].
+void* oracle_create(int precision, int nv, int nt, const float* rest, const int* tets,
+                    const double* params, int n_sdf, const int* sdf_kind, const int* sdf_fixed,
+                    const double* sdf_p, const double* sdf_dir, int threads) {
+  std::vector<SdfDesc> sd(n_sdf);
+  for (int i = 0; i < n_sdf; ++i) {
+    sd[i].kind = sdf_kind[i];
+    sd[i].fixed = sdf_fixed[i];
+    for (int k = 0; k < 4; ++k) sd[i].p[k] = sdf_p[4 * i + k];
+    for (int k = 0; k < 3; ++k) sd[i].dir[k] = sdf_dir[3 * i + k];
+  }
+  const Params prm = params_from(params);
+  Handle* h = new Handle;
+  h->precision = precision;
+  if (threads < 1) threads = 1;
+  if (precision == 32)
+    h->s32 = new Solver<float>(nv, nt, rest, tets, prm, sd.data(), n_sdf, threads);
+  else
+    h->s64 = new Solver<double>(nv, nt, rest, tets, prm, sd.data(), n_sdf, threads);
+  return h;
+}
+
+void oracle_destroy(void* hp) {
+  Handle* h = static_cast<Handle*>(hp);
+  delete h->s32;
+  delete h->s64;
+  delete h;
+}
+
+void oracle_frame(void* hp, int n) {
+  Handle* h = static_cast<Handle*>(hp);
+  for (int i = 0; i < n; ++i) dispatch(h, [](auto& s) { s.solve_frame(); return 0; });
+}
+
+void oracle_substep(void* hp, double sub_dt) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.substep(sub_dt); return 0; });
+}
+
+void oracle_get_x(void* hp, double* out) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.get_x(out); return 0; });
+}
+void oracle_set_x(void* hp, const double* in) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_x(in); return 0; });
+}
+void oracle_get_v(void* hp, double* out) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.get_v(out); return 0; });
+}
+void oracle_set_v(void* hp, const double* in) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_v(in); return 0; });
+}
+void oracle_get_pred(void* hp, double* out) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.get_pred(out); return 0; });
+}
+void oracle_get_mass(void* hp, double* m, double* inv_m) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.get_mass(m, inv_m); return 0; });
+}
+void oracle_get_tet_state(void* hp, double* v0, double* inv_dm, double* lam_d, double* lam_h) {
+  dispatch(static_cast<Handle*>(hp),
+           [&](auto& s) { s.get_tet_state(v0, inv_dm, lam_d, lam_h); return 0; });
+}
+void oracle_set_gravity(void* hp, const double* g) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_gravity(g); return 0; });
+}
+void oracle_compute_lumped_mass(void* hp, double total_mass) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) {
+    s.set_total_mass(total_mass);
+    s.compute_lumped_mass();
+    return 0;
+  });
+}
+double oracle_rho_hat(void* hp) {
+  return dispatch(static_cast<Handle*>(hp), [](auto& s) { return s.rho_hat(); });
+}
+void oracle_set_log(void* hp, int enabled) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.log_enabled_ = enabled != 0; return 0; });
+}
+int oracle_log_size(void* hp) {
+  return dispatch(static_cast<Handle*>(hp), [](auto& s) { return int(s.log().size()); });
+}
+// rows of 8 doubles: frame, substep, iter, r_hat, rho_hat, omega, hydro_error, dev_error
+void oracle_get_log(void* hp, double* out) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) {
+    size_t i = 0;
+    for (const auto& r : s.log()) {
+      out[8 * i + 0] = r.frame;
+      out[8 * i + 1] = r.substep;
+      out[8 * i + 2] = r.iter;
+      out[8 * i + 3] = r.r_hat;
+      out[8 * i + 4] = r.rho_hat;
+      out[8 * i + 5] = r.omega;
+      out[8 * i + 6] = r.hydro_error;
+      out[8 * i + 7] = r.dev_error;
+      ++i;
+    }
+    return 0;
+  });
+}
+void oracle_clear_log(void* hp) {
+  dispatch(static_cast<Handle*>(hp), [](auto& s) { s.clear_log(); return 0; });
+}
+
+// ---- single-tet entry points (for closed-form and finite-difference pins) ----
+
+// Block projection of one tet in isolation.  p: 4x3 positions, w: 4 inverse
+// masses, inv_dm: 3x3 row-major, lam: [lambda_D, lambda_H] in/out, out
+// corner: 4x3 corrections.  aux (8): C_D, C_H, A11, A12, A22, dlamD, dlamH, ok.
+void oracle_block_tet64(const double* p, const double* w, const double* inv_dm, double v0,
+                        double lame1, double lame2, double dt, double* lam, double* corner,
+                        double* grads, double* aux) {
+  M3<double> idm;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) idm.m[i][j] = inv_dm[3 * i + j];
+  const M3<double> F = deformation_gradient(&p[0], &p[3], &p[6], &p[9], idm);
+  const double C_D = norm_sqr(F) - 3.0;
+  const double C_H = det3(F) - 1.0;
+  M3<double> gfd;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * 2.0;
+  double gD[4][3], gH[4][3];
+  vertex_grads(gfd, idm, gD);
+  vertex_grads(ddet_dF(F), idm, gH);
+  double A11 = 0, A22 = 0, A12 = 0;
+  for (int k = 0; k < 4; ++k) {
+    A11 += w[k] * dot3(gD[k], gD[k]);
+    A22 += w[k] * dot3(gH[k], gH[k]);
+    A12 += w[k] * dot3(gD[k], gH[k]);
+  }
+  const double aD = (1.0 / (lame2 * v0)) / (dt * dt);
+  const double aH = (1.0 / (lame1 * v0)) / (dt * dt);
+  A11 += aD;
+  A22 += aH;
+  const double b1 = -(C_D + aD * lam[0]);
+  const double b2 = -(C_H + aH * lam[1]);
+  double dlD, dlH;
+  const int ok = solve2x2_pivot_lu(A11, A12, A12, A22, b1, b2, kEpsilon, &dlD, &dlH);
+  if (ok == 0) {
+    dlD = 0.0;
+    dlH = 0.0;
+    if (std::fabs(A11) > kEpsilon) dlD = b1 / A11;
+    if (std::fabs(A22) > kEpsilon) dlH = b2 / A22;
+  }
+  lam[0] += dlD;
+  lam[1] += dlH;
+  for (int k = 0; k < 4; ++k)
+    for (int r = 0; r < 3; ++r) {
+      corner[3 * k + r] = w[k] * (gD[k][r] * dlD + gH[k][r] * dlH);
+      if (grads) {
+        grads[3 * k + r] = gD[k][r];
+        grads[12 + 3 * k + r] = gH[k][r];
+      }
+    }
+  if (aux) {
+    aux[0] = C_D;
+    aux[1] = C_H;
+    aux[2] = A11;
+    aux[3] = A12;
+    aux[4] = A22;
+    aux[5] = dlD;
+    aux[6] = dlH;
+    aux[7] = ok;
+  }
+}
+
+int oracle_solve2x2_64(double a00, double a01, double a10, double a11, double b0, double b1,
+                       double eps, double* x) {
+  return solve2x2_pivot_lu(a00, a01, a10, a11, b0, b1, eps, &x[0], &x[1]);
+}
+
+// inverse and determinant of a row-major 3x3 as the oracle evaluates them
+void oracle_inv3_64(const double* a, double* out, double* det) {
+  M3<double> m;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) m.m[i][j] = a[3 * i + j];
+  const M3<double> r = inv3(m);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) out[3 * i + j] = r.m[i][j];
+  *det = det3(m);
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// Device-side data structures and per-tet math shared by the gfx950 kernels.
+//
+// Layout in HBM (one scene):
+//   vertex streams, float4 per vertex (16 B, one 64 B line holds 4 vertices):
+//     pos   pred_x.xyz | inv_m          gathered by every tet, so the inverse mass
+//                                       rides in the same 16 B record
+//     xs    x.xyz      | inv_m
+//     vel   v.xyz      | 0
+//     prev  Chebyshev y_prev
+//     xj    Jacobi-averaged positions of the current iteration
+//     dacc  delta_x.xyz | cons_num      (float atomics)
+//   tet streams (SoA over tets, coalesced per wavefront):
+//     tet   int4 vertex ids
+//     dm0   float4 (m00 m01 m02 m10) of inv_Dm, row-major
+//     dm1   float4 (m11 m12 m20 m21)
+//     dm2   float2 (m22, V0)
+//     lam   float2 (lambda_D, lambda_H)
+//   deterministic mode: corner float4[4T] per tet-corner corrections and a
+//   vertex -> corner CSR sorted by tet.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/xpbd_abi.h"
+
+namespace xpbd {
+
+struct SdfDev {
+  int kind;   // XPBD_SDF_PLANE / XPBD_SDF_SPHERE
+  int fixed;
+  float p[3];
+  float r;
+  float d[3];
+  float pad;
+};
+
+// Per-scene scalar state that persists across launches (rho_hat persists over
+// the solver's life, neohookean_solver.py:65; the rest is per substep).
+struct SceneCtrl {
+  double rho_hat;
+  double prev_r_hat;
+  double omega;
+  double r_hat;
+  int have_prev;
+  int status;
+  int pad[2];
+};
+
+struct SceneDev {
+  float4* pos;
+  float4* xs;
+  float4* vel;
+  float4* prev;
+  float4* xj;
+  float4* dacc;
+  const float* valence;
+  const int* adj_off;
+  const int* adj;
+  float4* corner;
+  const int4* tet;
+  const float4* dm0;
+  const float4* dm1;
+  const float2* dm2;
+  float2* lam;
+  double* partial;  // 3 doubles per tet block: sum r, sum |C_H|, sum |C_D|
+  unsigned* ticket;
+  SceneCtrl* ctrl;
+  double* log;      // 6 doubles per iteration: iter, r_hat, rho_hat, omega, hydro, dev
+  int n_verts;
+  int n_tets;
+  int n_tet_blocks;
+  int tet_block;    // threads per tet block (partial index = first_tet / tet_block)
+  // parameters as the reference kernels see them (f32 constants / arguments)
+  float lame1, lame2;
+  float dt;
+  float gamma;
+  float hydro_relax, dev_relax;
+  float decay;      // exp(-dt * damping), evaluated once on the host in f32
+  float g[3];
+  int block_mode;   // block_neohookean_enable
+  int cheb;         // cheb_enable
+  int monitor;      // monitor_convergence
+  int csr;          // deterministic accumulation
+  int warmup;
+  int iters;
+  int n_sdf;
+  int pad;
+  SdfDev sdf[XPBD_MAX_SDF];
+};
+
+// Work map entry: which scene a workgroup belongs to and its first element.
+struct WorkItem {
+  int scene;
+  int first;
+};
+
+constexpr float kEps = 1e-12f;  // xpbd_gpu/constants.py:3 as an f32 constant
+
+__device__ __forceinline__ float3 f3(const float4& a) { return make_float3(a.x, a.y, a.z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) {
+  return make_float3(a.x + b.x, a.y + b.y, a.z + b.z);
+}
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) {
+  return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+}
+__device__ __forceinline__ float3 operator*(float s, float3 a) {
+  return make_float3(s * a.x, s * a.y, s * a.z);
+}
+__device__ __forceinline__ float3 operator*(float3 a, float s) {
+  return make_float3(a.x * s, a.y * s, a.z * s);
+}
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Row-major 3x3 held in registers.
+struct Mat3 {
+  float m[3][3];
+};
+
+__device__ __forceinline__ Mat3 load_inv_dm(const float4& a, const float4& b, const float2& c) {
+  Mat3 r;
+  r.m[0][0] = a.x; r.m[0][1] = a.y; r.m[0][2] = a.z;
+  r.m[1][0] = a.w; r.m[1][1] = b.x; r.m[1][2] = b.y;
+  r.m[2][0] = b.z; r.m[2][1] = b.w; r.m[2][2] = c.x;
+  return r;
+}
+
+// F = Ds @ inv_Dm, Ds = cols(p1-p0, p2-p0, p3-p0)  (neohookean_solver.py:371-373)
+__device__ __forceinline__ Mat3 deformation_gradient(float3 p0, float3 p1, float3 p2, float3 p3,
+                                                     const Mat3& idm) {
+  const float3 e1 = p1 - p0, e2 = p2 - p0, e3 = p3 - p0;
+  const float ds[3][3] = {{e1.x, e2.x, e3.x}, {e1.y, e2.y, e3.y}, {e1.z, e2.z, e3.z}};
+  Mat3 f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      f.m[i][j] = ds[i][0] * idm.m[0][j] + ds[i][1] * idm.m[1][j] + ds[i][2] * idm.m[2][j];
+  return f;
+}
+
+__device__ __forceinline__ float norm_sqr(const Mat3& a) {
+  float s = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s += a.m[i][j] * a.m[i][j];
+  return s;
+}
+
+// cofactor expansion along the first column
+__device__ __forceinline__ float det3(const Mat3& a) {
+  const auto& m = a.m;
+  return m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+         m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+         m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+}
+
+// Vertex gradients g1..g3 = columns of dCdF @ inv_Dm^T, g0 = -(g1+g2+g3)
+// (neohookean_solver.py:382-404).
+__device__ __forceinline__ void vertex_grads(const Mat3& dcdf, const Mat3& idm, float3 g[4]) {
+  float c[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      c[i][j] = dcdf.m[i][0] * idm.m[j][0] + dcdf.m[i][1] * idm.m[j][1] + dcdf.m[i][2] * idm.m[j][2];
+  g[1] = make_float3(c[0][0], c[1][0], c[2][0]);
+  g[2] = make_float3(c[0][1], c[1][1], c[2][1]);
+  g[3] = make_float3(c[0][2], c[1][2], c[2][2]);
+  g[0] = make_float3(-(g[1].x + g[2].x + g[3].x), -(g[1].y + g[2].y + g[3].y),
+                     -(g[1].z + g[2].z + g[3].z));
+}
+
+// dC_H/dF = cols(F1 x F2, F2 x F0, F0 x F1)  (neohookean_solver.py:391-398)
+__device__ __forceinline__ Mat3 ddet_dF(const Mat3& f) {
+  const float3 c0 = make_float3(f.m[0][0], f.m[1][0], f.m[2][0]);
+  const float3 c1 = make_float3(f.m[0][1], f.m[1][1], f.m[2][1]);
+  const float3 c2 = make_float3(f.m[0][2], f.m[1][2], f.m[2][2]);
+  const float3 x0 = cross(c1, c2), x1 = cross(c2, c0), x2 = cross(c0, c1);
+  Mat3 g;
+  g.m[0][0] = x0.x; g.m[1][0] = x0.y; g.m[2][0] = x0.z;
+  g.m[0][1] = x1.x; g.m[1][1] = x1.y; g.m[2][1] = x1.z;
+  g.m[0][2] = x2.x; g.m[1][2] = x2.y; g.m[2][2] = x2.z;
+  return g;
+}
+
+// solve2x2_pivot_lu (neohookean_solver.py:331-356) with the diagonal fallback
+// of :425-432.
+__device__ __forceinline__ void solve2x2_with_fallback(float A11, float A12, float A22, float b1,
+                                                       float b2, float& x0, float& x1) {
+  float a00 = A11, a01 = A12, a10 = A12, a11 = A22, c0 = b1, c1 = b2;
+  int ok = 1;
+  if (fabsf(a00) < fabsf(a10)) {
+    float t;
+    t = a00; a00 = a10; a10 = t;
+    t = a01; a01 = a11; a11 = t;
+    t = c0; c0 = c1; c1 = t;
+  }
+  if (fabsf(a00) < kEps) ok = 0;
+  x0 = 0.f;
+  x1 = 0.f;
+  if (ok) {
+    const float m = a10 / a00;
+    const float a11_ = a11 - m * a01;
+    const float c1_ = c1 - m * c0;
+    if (fabsf(a11_) < kEps) {
+      ok = 0;
+    } else {
+      x1 = c1_ / a11_;
+      x0 = (c0 - a01 * x1) / a00;
+    }
+  }
+  if (!ok) {
+    x0 = 0.f;
+    x1 = 0.f;
+    if (fabsf(A11) > kEps) x0 = b1 / A11;
+    if (fabsf(A22) > kEps) x1 = b2 / A22;
+  }
+}
+
+// SdfModel.check + dist + normal for PlaneSdf / SphereSdf (sdf.py:13-27, 59-95).
+__device__ __forceinline__ float sdf_dist(const SdfDev& s, float3 x) {
+  const float3 q = x - make_float3(s.p[0], s.p[1], s.p[2]);
+  if (s.kind == XPBD_SDF_PLANE) return dot(q, make_float3(s.d[0], s.d[1], s.d[2]));
+  return sqrtf(dot(q, q)) - s.r;
+}
+
+__device__ __forceinline__ float3 sdf_normal(const SdfDev& s, float3 x) {
+  if (s.kind == XPBD_SDF_PLANE) return make_float3(s.d[0], s.d[1], s.d[2]);
+  const float3 q = x - make_float3(s.p[0], s.p[1], s.p[2]);
+  const float inv_len = 1.f / (sqrtf(dot(q, q)) + kEps);
+  return inv_len * q;
+}
+
+// One collider projection of one vertex (xpbd_base.py:162-174).
+__device__ __forceinline__ float3 sdf_project(const SdfDev& s, float3 x, float3 v) {
+  const float phi = sdf_dist(s, x);
+  if (phi < 0.f) {
+    const float3 n = sdf_normal(s, x);
+    const float3 dv = make_float3(0.f - v.x, 0.f - v.y, 0.f - v.z);
+    const float dotnv = dot(n, dv);
+    if (dotnv > 0.f || s.fixed) {
+      const float depth = -sdf_dist(s, x);
+      if (depth > 0.f) x = x + n * depth;
+    }
+  }
+  return x;
+}
+
+}  // namespace xpbd

@@ -1,0 +1,1037 @@
+// Host side of libxpbd_hip.so: the C ABI of include/xpbd_abi.h, mesh set-up
+// (reordering, rest-shape data, lumped masses, adjacency), and the launch
+// sequence of a substep (one kernel per phase, optionally captured in a
+// hipGraph).
+//
+// Set-up arithmetic restates the reference's f32 initialisation kernels
+// (neohookean_solver.py:97-107, xpbd_base.py:89-137) on the host.  It is
+// compiled without floating-point contraction so that each f32 operation is
+// rounded as written.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/xpbd_abi.h"
+#include "xpbd_device.h"
+#include "xpbd_kernels.h"
+
+using xpbd::SceneCtrl;
+using xpbd::SceneDev;
+using xpbd::SdfDev;
+using xpbd::WorkItem;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int set_error(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return set_error(e_ == hipErrorOutOfMemory ? XPBD_ENOMEM : XPBD_EHIP, "%s: %s (%s:%d)", \
+                       #expr, hipGetErrorString(e_), __FILE__, __LINE__);                \
+  } while (0)
+
+#define TRY(expr)               \
+  do {                          \
+    int r_ = (expr);            \
+    if (r_ != XPBD_OK) return r_; \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  int alloc(size_t count) {
+    release();
+    n = count;
+    if (count == 0) return XPBD_OK;
+    HIP_TRY(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    return XPBD_OK;
+  }
+  int upload(const T* src, size_t count) {
+    TRY(alloc(count));
+    if (count) HIP_TRY(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return XPBD_OK;
+  }
+  int zero() {
+    if (n) HIP_TRY(hipMemset(p, 0, n * sizeof(T)));
+    return XPBD_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+// 10-bit-per-axis Morton code of a point inside [lo, hi].
+uint32_t morton3(const float* x, const float* lo, const float* ext) {
+  uint32_t code = 0;
+  uint32_t q[3];
+  for (int k = 0; k < 3; ++k) {
+    float u = ext[k] > 0.f ? (x[k] - lo[k]) / ext[k] : 0.f;
+    u = std::min(std::max(u, 0.f), 1.f);
+    q[k] = std::min<uint32_t>(1023u, uint32_t(u * 1023.f));
+  }
+  for (int b = 9; b >= 0; --b)
+    for (int k = 0; k < 3; ++k) code = (code << 1) | ((q[k] >> b) & 1u);
+  return code;
+}
+
+float tet_volume_f32(const float* p0, const float* p1, const float* p2, const float* p3) {
+  // calc_tet_volume (xpbd_base.py:6-8): (p1-p0) x (p2-p0) . (p3-p0) / 6
+  const float a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+  const float b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+  const float c[3] = {p3[0] - p0[0], p3[1] - p0[1], p3[2] - p0[2]};
+  const float ab[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                       a[0] * b[1] - a[1] * b[0]};
+  return (ab[0] * c[0] + ab[1] * c[1] + ab[2] * c[2]) / 6.f;
+}
+
+// Matrix.inverse for 3x3: adjugate times 1/det, det by first-column expansion.
+void inv3_f32(const float m[3][3], float out[3][3]) {
+  const float det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                    m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+                    m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+  const float inv_det = 1.f / det;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      out[j][i] = inv_det * (m[i1][j1] * m[i2][j2] - m[i2][j1] * m[i1][j2]);
+    }
+}
+
+SdfDev to_dev(const xpbd_sdf& s) {
+  SdfDev d;
+  std::memset(&d, 0, sizeof(d));
+  d.kind = s.kind;
+  d.fixed = s.fixed;
+  for (int k = 0; k < 3; ++k) d.p[k] = float(s.p[k]);
+  d.r = float(s.p[3]);
+  // PlaneSdf normalizes dir at construction in Python (sdf.py:66).
+  const double n = std::sqrt(s.dir[0] * s.dir[0] + s.dir[1] * s.dir[1] + s.dir[2] * s.dir[2]);
+  for (int k = 0; k < 3; ++k) d.d[k] = n > 0 ? float(s.dir[k] / n) : 0.f;
+  return d;
+}
+
+// Host copy of SdfModel.check (sdf.py:13-27) used by the fixed-vertex pinning
+// of compute_lumped_mass (xpbd_base.py:124-131).
+bool sdf_inside_host(const SdfDev& s, const float* x, const float* v) {
+  const float q[3] = {x[0] - s.p[0], x[1] - s.p[1], x[2] - s.p[2]};
+  float phi, n[3];
+  if (s.kind == XPBD_SDF_PLANE) {
+    phi = q[0] * s.d[0] + q[1] * s.d[1] + q[2] * s.d[2];
+  } else {
+    phi = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) - s.r;
+  }
+  if (!(phi < 0.f)) return false;
+  if (s.kind == XPBD_SDF_PLANE) {
+    for (int k = 0; k < 3; ++k) n[k] = s.d[k];
+  } else {
+    const float inv_len = 1.f / (std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) + 1e-12f);
+    for (int k = 0; k < 3; ++k) n[k] = inv_len * q[k];
+  }
+  const float dv[3] = {0.f - v[0], 0.f - v[1], 0.f - v[2]};
+  const float dotnv = n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2];
+  return dotnv > 0.f || s.fixed;
+}
+
+int validate_sdfs(const xpbd_sdf* sdfs, int n) {
+  if (n < 1) return set_error(XPBD_EINVAL, "At least one SDF object must be provided.");
+  if (n > XPBD_MAX_SDF) return set_error(XPBD_EINVAL, "at most %d SDFs are supported", XPBD_MAX_SDF);
+  if (!sdfs) return set_error(XPBD_EINVAL, "sdfs is NULL");
+  for (int i = 0; i < n; ++i)
+    if (sdfs[i].kind != XPBD_SDF_PLANE && sdfs[i].kind != XPBD_SDF_SPHERE)
+      return set_error(XPBD_EINVAL, "sdf %d: unknown kind %d", i, sdfs[i].kind);
+  return XPBD_OK;
+}
+
+}  // namespace
+
+struct xpbd_solver {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  xpbd_desc prm{};                 // scalar parameters (mesh pointers cleared)
+  std::vector<xpbd_sdf> sdfs;
+  int nv = 0, nt = 0;
+  int bs_t = 256, bs_v = 256;
+  std::vector<int> v_new2old, v_old2new, t_new2old;
+  std::vector<int> tets_orig;      // caller order, for the visual indices
+  std::vector<int> tets_new;       // internal order, internal vertex ids
+  std::vector<float> m_host, inv_m_host;  // internal order
+
+  DevBuf<float4> pos, xs, vel, prev, xj, dacc, corner;
+  DevBuf<float> valence;
+  DevBuf<int> adj_off, adj, d_old2new;
+  DevBuf<int4> tet;
+  DevBuf<float4> dm0, dm1;
+  DevBuf<float2> dm2, lam;
+  DevBuf<double> partial, logbuf;
+  DevBuf<unsigned> ticket;
+  DevBuf<SceneCtrl> ctrl;
+  DevBuf<SceneDev> table;
+  DevBuf<WorkItem> vmap, tmap;
+  DevBuf<float> scratch_xyz;
+  DevBuf<int> flag;
+  int nvb = 0, ntb = 0;
+
+  SceneDev host_scene{};
+  bool table_valid = false;
+
+  int frame = 0, substep = 0;
+  double time = 0.0;
+  bool logging = false;
+  std::vector<double> log_rows;
+  double* log_host = nullptr;     // pinned staging for one substep's rows
+
+  int path = XPBD_PATH_GRAPH;
+  hipGraphExec_t gexec = nullptr;
+  int graph_key = -1;
+
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
+  double timed_ms = 0.0;
+  int64_t timed_launches = 0;
+
+  ~xpbd_solver() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (auto& e : ev_pending) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    for (auto& e : ev_free) {
+      (void)hipEventDestroy(e.first);
+      (void)hipEventDestroy(e.second);
+    }
+    if (log_host) (void)hipHostFree(log_host);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int validate_params(const xpbd_desc* d) {
+  if (d->substeps_num < 1) return set_error(XPBD_EINVAL, "substeps_num must be >= 1");
+  if (d->constraints_iter < 0) return set_error(XPBD_EINVAL, "constraints_iter must be >= 0");
+  if (d->block_size < 64 || d->block_size > 1024 || d->block_size % 64)
+    return set_error(XPBD_EINVAL, "block_size must be a multiple of 64 in [64, 1024]");
+  if (d->accumulate_mode != XPBD_ACCUM_ATOMIC && d->accumulate_mode != XPBD_ACCUM_CSR)
+    return set_error(XPBD_EINVAL, "unknown accumulate_mode %d", d->accumulate_mode);
+  if (!(d->frame_dt > 0) || !(d->dt > 0)) return set_error(XPBD_EINVAL, "frame_dt and dt must be > 0");
+  return XPBD_OK;
+}
+
+// Parameters as the reference kernels see them: Python doubles frozen into
+// f32 constants (neohookean_solver.py:7-12, 43-44).
+void fill_scene_params(xpbd_solver* s, SceneDev& S, double dt, double damping) {
+  const xpbd_desc& d = s->prm;
+  const double E = d.young_modulus, nu = d.poisson_ratio;
+  const double mu = E / (2.0 * (1.0 + nu));
+  const double la = 2.0 * nu / (1.0 - 2.0 * nu) * mu;
+  S.lame1 = float(la);
+  S.lame2 = float(mu);
+  S.dt = float(dt);
+  S.gamma = float(d.cheb_gamma);
+  S.hydro_relax = float(d.hydrostatic_relaxation);
+  S.dev_relax = float(d.deviatoric_relaxation);
+  S.decay = std::exp(-S.dt * float(damping));  // ti.exp(-dt * damping) in f32
+  for (int k = 0; k < 3; ++k) S.g[k] = float(d.gravity[k]);
+  S.block_mode = d.block_neohookean_enable ? 1 : 0;
+  S.cheb = d.cheb_enable ? 1 : 0;
+  S.monitor = d.monitor_convergence ? 1 : 0;
+  S.csr = d.accumulate_mode == XPBD_ACCUM_CSR ? 1 : 0;
+  S.warmup = d.cheb_warmup;
+  S.iters = d.constraints_iter;
+  S.n_sdf = int(s->sdfs.size());
+  for (int i = 0; i < S.n_sdf; ++i) S.sdf[i] = to_dev(s->sdfs[i]);
+}
+
+int build_scene(xpbd_solver* s) {
+  SceneDev& S = s->host_scene;
+  std::memset(&S, 0, sizeof(S));
+  S.pos = s->pos.p;
+  S.xs = s->xs.p;
+  S.vel = s->vel.p;
+  S.prev = s->prev.p;
+  S.xj = s->xj.p;
+  S.dacc = s->dacc.p;
+  S.valence = s->valence.p;
+  S.adj_off = s->adj_off.p;
+  S.adj = s->adj.p;
+  S.corner = s->corner.p;
+  S.tet = s->tet.p;
+  S.dm0 = s->dm0.p;
+  S.dm1 = s->dm1.p;
+  S.dm2 = s->dm2.p;
+  S.lam = s->lam.p;
+  S.partial = s->partial.p;
+  S.ticket = s->ticket.p;
+  S.ctrl = s->ctrl.p;
+  S.log = s->logbuf.p;
+  S.n_verts = s->nv;
+  S.n_tets = s->nt;
+  S.n_tet_blocks = s->ntb;
+  S.tet_block = s->bs_t;
+  fill_scene_params(s, S, s->prm.dt / s->prm.substeps_num, s->prm.damping);
+  s->table_valid = false;
+  return XPBD_OK;
+}
+
+// Uploads the scene table if (dt, damping) or any parameter changed.
+int sync_table(xpbd_solver* s, double dt, double damping) {
+  SceneDev S = s->host_scene;
+  fill_scene_params(s, S, dt, damping);
+  if (s->table_valid && std::memcmp(&S, &s->host_scene, sizeof(S)) == 0) return XPBD_OK;
+  s->host_scene = S;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(s->table.p, &s->host_scene, sizeof(SceneDev), hipMemcpyHostToDevice));
+  s->table_valid = true;
+  return XPBD_OK;
+}
+
+int build_maps(xpbd_solver* s) {
+  std::vector<WorkItem> vm, tm;
+  for (int v = 0; v < s->nv; v += s->bs_v) vm.push_back({0, v});
+  for (int t = 0; t < s->nt; t += s->bs_t) tm.push_back({0, t});
+  s->nvb = int(vm.size());
+  s->ntb = int(tm.size());
+  TRY(s->vmap.upload(vm.data(), vm.size()));
+  TRY(s->tmap.upload(tm.data(), tm.size()));
+  return XPBD_OK;
+}
+
+// _compute_lumped_mass_kernel + _apply_fixed_vertices_for_sdf
+// (xpbd_base.py:89-137) on host copies of x and v (internal order); the
+// atomic accumulation order of the reference is unspecified, this sums in
+// ascending internal tet order.
+void lumped_mass_host(xpbd_solver* s, const std::vector<float>& x, const std::vector<float>& v,
+                      double total_mass) {
+  s->m_host.assign(s->nv, 0.f);
+  s->inv_m_host.assign(s->nv, 0.f);
+  float total_volume = 0.f;
+  for (int t = 0; t < s->nt; ++t) {
+    const int* tv = &s->tets_new[4 * size_t(t)];
+    const float vol = tet_volume_f32(&x[3 * size_t(tv[0])], &x[3 * size_t(tv[1])],
+                                     &x[3 * size_t(tv[2])], &x[3 * size_t(tv[3])]);
+    const float quarter = vol * 0.25f;
+    for (int k = 0; k < 4; ++k) s->m_host[tv[k]] += quarter;
+    total_volume += vol;
+  }
+  const float mass_scale = float(total_mass) / total_volume;
+  for (int i = 0; i < s->nv; ++i) {
+    s->m_host[i] *= mass_scale;
+    s->inv_m_host[i] = s->m_host[i] > 0.f ? 1.f / s->m_host[i] : 0.f;
+  }
+  for (const auto& sd : s->sdfs) {
+    const SdfDev d = to_dev(sd);
+    for (int i = 0; i < s->nv; ++i)
+      if (s->inv_m_host[i] > 0.f && d.fixed && sdf_inside_host(d, &x[3 * size_t(i)], &v[3 * size_t(i)]))
+        s->inv_m_host[i] = 0.f;
+  }
+}
+
+int download_vec4_xyz(xpbd_solver* s, const DevBuf<float4>& b, std::vector<float>& out,
+                      std::vector<float>* w = nullptr) {
+  std::vector<float4> tmp(s->nv);
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->nv) HIP_TRY(hipMemcpy(tmp.data(), b.p, sizeof(float4) * s->nv, hipMemcpyDeviceToHost));
+  out.resize(3 * size_t(s->nv));
+  if (w) w->resize(s->nv);
+  for (int i = 0; i < s->nv; ++i) {
+    out[3 * i] = tmp[i].x;
+    out[3 * i + 1] = tmp[i].y;
+    out[3 * i + 2] = tmp[i].z;
+    if (w) (*w)[i] = tmp[i].w;
+  }
+  return XPBD_OK;
+}
+
+int upload_inv_m(xpbd_solver* s) {
+  // inv_m lives in the w lane of pos and xs
+  std::vector<float4> tmp(s->nv);
+  for (auto* b : {&s->pos, &s->xs}) {
+    HIP_TRY(hipMemcpy(tmp.data(), b->p, sizeof(float4) * s->nv, hipMemcpyDeviceToHost));
+    for (int i = 0; i < s->nv; ++i) tmp[i].w = s->inv_m_host[i];
+    HIP_TRY(hipMemcpy(b->p, tmp.data(), sizeof(float4) * s->nv, hipMemcpyHostToDevice));
+  }
+  return XPBD_OK;
+}
+
+int events_begin(xpbd_solver* s, std::pair<hipEvent_t, hipEvent_t>& ev) {
+  if (!s->ev_free.empty()) {
+    ev = s->ev_free.back();
+    s->ev_free.pop_back();
+  } else {
+    HIP_TRY(hipEventCreate(&ev.first));
+    HIP_TRY(hipEventCreate(&ev.second));
+  }
+  HIP_TRY(hipEventRecord(ev.first, s->stream));
+  return XPBD_OK;
+}
+
+int events_end(xpbd_solver* s, std::pair<hipEvent_t, hipEvent_t>& ev) {
+  HIP_TRY(hipEventRecord(ev.second, s->stream));
+  s->ev_pending.push_back(ev);
+  return XPBD_OK;
+}
+
+int events_collect(xpbd_solver* s) {
+  for (auto& ev : s->ev_pending) {
+    HIP_TRY(hipEventSynchronize(ev.second));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second));
+    s->timed_ms += ms;
+    s->timed_launches += 1;
+    s->ev_free.push_back(ev);
+  }
+  s->ev_pending.clear();
+  return XPBD_OK;
+}
+
+// The Jacobi iterations of solve_step (neohookean_solver.py:474-549).
+int enqueue_iterations(xpbd_solver* s, bool capturing) {
+  const SceneDev& S = s->host_scene;
+  const SceneDev* T = s->table.p;
+  for (int it = 0; it < S.iters; ++it) {
+    std::pair<hipEvent_t, hipEvent_t> ev;
+    const bool timed = s->timing && !capturing;
+    if (timed) TRY(events_begin(s, ev));
+    HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
+    if (timed) TRY(events_end(s, ev));
+    HIP_TRY(xpbd::launch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
+    if (S.cheb || S.monitor)
+      HIP_TRY(xpbd::launch_residual(T, s->tmap.p, s->ntb, s->bs_t, it, s->stream));
+    if (S.cheb) HIP_TRY(xpbd::launch_blend(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
+  }
+  return XPBD_OK;
+}
+
+int enqueue_solve_step(xpbd_solver* s, bool capturing) {
+  // lambda reset happens in the first projection; collision then save prev
+  HIP_TRY(xpbd::launch_predict(s->table.p, s->vmap.p, s->nvb, s->bs_v, 0, 1,
+                               s->host_scene.cheb, s->stream));
+  return enqueue_iterations(s, capturing);
+}
+
+int enqueue_substep(xpbd_solver* s, bool capturing) {
+  HIP_TRY(xpbd::launch_predict(s->table.p, s->vmap.p, s->nvb, s->bs_v, 1, 1,
+                               s->host_scene.cheb, s->stream));
+  TRY(enqueue_iterations(s, capturing));
+  HIP_TRY(xpbd::launch_advance(s->table.p, s->vmap.p, s->nvb, s->bs_v, s->stream));
+  return XPBD_OK;
+}
+
+int graph_key_of(const SceneDev& S) {
+  return (S.iters << 4) | (S.cheb << 0) | (S.monitor << 1) | (S.block_mode << 2) | (S.csr << 3);
+}
+
+int fetch_log(xpbd_solver* s) {
+  if (!s->logging) return XPBD_OK;
+  const int iters = s->host_scene.iters;
+  if (iters == 0) return XPBD_OK;
+  HIP_TRY(hipMemcpyAsync(s->log_host, s->logbuf.p, sizeof(double) * 6 * iters,
+                         hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (int it = 0; it < iters; ++it) {
+    const double* r = s->log_host + 6 * it;
+    const double row[8] = {double(s->frame), double(s->substep), r[0], r[1], r[2], r[3], r[4], r[5]};
+    s->log_rows.insert(s->log_rows.end(), row, row + 8);
+  }
+  return XPBD_OK;
+}
+
+int run_substep(xpbd_solver* s, double sub_dt) {
+  TRY(sync_table(s, sub_dt, s->prm.damping));
+  if (s->path == XPBD_PATH_GRAPH && !s->timing) {
+    const int key = graph_key_of(s->host_scene);
+    if (!s->gexec || key != s->graph_key) {
+      if (s->gexec) {
+        HIP_TRY(hipGraphExecDestroy(s->gexec));
+        s->gexec = nullptr;
+      }
+      hipGraph_t g;
+      HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_substep(s, true);
+      const hipError_t ec = hipStreamEndCapture(s->stream, &g);
+      if (rc != XPBD_OK) return rc;
+      HIP_TRY(ec);
+      const hipError_t ei = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ei);
+      s->graph_key = key;
+    }
+    HIP_TRY(hipGraphLaunch(s->gexec, s->stream));
+  } else {
+    TRY(enqueue_substep(s, false));
+  }
+  return fetch_log(s);
+}
+
+int set_positions_common(xpbd_solver* s, const float* src_dev) {
+  HIP_TRY(xpbd::launch_scatter_xyz(s->xs.p, src_dev, s->d_old2new.p, s->nv, 1, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return XPBD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xpbd_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return XPBD_EINVAL;
+  std::snprintf(buf, len, "%s", g_last_error.c_str());
+  return XPBD_OK;
+}
+
+int xpbd_abi_info(int32_t* version, size_t* desc_size, size_t* sdf_size) {
+  if (version) *version = XPBD_ABI_VERSION;
+  if (desc_size) *desc_size = sizeof(xpbd_desc);
+  if (sdf_size) *sdf_size = sizeof(xpbd_sdf);
+  return XPBD_OK;
+}
+
+int xpbd_desc_init(xpbd_desc* d) {
+  if (!d) return set_error(XPBD_EINVAL, "desc is NULL");
+  std::memset(d, 0, sizeof(*d));
+  d->young_modulus = 200000.0;
+  d->poisson_ratio = 0.495;
+  d->scale = 1.0;
+  d->frame_dt = 1e-2;
+  d->dt = 1e-2;
+  d->substeps_num = 20;
+  d->constraints_iter = 5;
+  d->hydrostatic_relaxation = 1.2;
+  d->deviatoric_relaxation = 1.2;
+  d->cheb_enable = 1;
+  d->cheb_warmup = 5;
+  d->cheb_gamma = 0.6666;
+  d->block_neohookean_enable = 0;
+  d->monitor_convergence = 0;
+  d->damping = 5.0;
+  d->gravity[1] = -9.8;
+  d->total_mass = 1.0;
+  d->accumulate_mode = XPBD_ACCUM_ATOMIC;
+  d->reorder = 1;
+  d->block_size = 256;
+  d->device = 0;
+  return XPBD_OK;
+}
+
+int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
+  if (!d || !out) return set_error(XPBD_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (d->n_verts <= 0 || d->n_tets <= 0 || !d->rest_positions || !d->tets)
+    return set_error(XPBD_EINVAL, "mesh must have vertices and tets");
+  TRY(validate_params(d));
+  TRY(validate_sdfs(d->sdfs, d->n_sdf));
+  const int nv = d->n_verts, nt = d->n_tets;
+  for (size_t i = 0; i < 4 * size_t(nt); ++i)
+    if (d->tets[i] < 0 || d->tets[i] >= nv)
+      return set_error(XPBD_EINVAL, "tet %zu references vertex %d outside [0, %d)", i / 4,
+                       d->tets[i], nv);
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (d->device < 0 || d->device >= ndev)
+    return set_error(XPBD_EINVAL, "device %d not present (%d visible)", d->device, ndev);
+  HIP_TRY(hipSetDevice(d->device));
+
+  auto s = std::make_unique<xpbd_solver>();
+  s->device = d->device;
+  s->prm = *d;
+  s->prm.rest_positions = nullptr;
+  s->prm.tets = nullptr;
+  s->prm.sdfs = nullptr;
+  s->sdfs.assign(d->sdfs, d->sdfs + d->n_sdf);
+  s->nv = nv;
+  s->nt = nt;
+  s->bs_t = d->block_size;
+  s->bs_v = 256;
+  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+
+  // initialize (neohookean_solver.py:99-100): x = x * scale + offset, in f32
+  const float scale = float(d->scale);
+  const float off[3] = {float(d->offset[0]), float(d->offset[1]), float(d->offset[2])};
+  std::vector<float> x0(3 * size_t(nv));
+  for (int i = 0; i < nv; ++i)
+    for (int k = 0; k < 3; ++k) x0[3 * size_t(i) + k] = d->rest_positions[3 * size_t(i) + k] * scale + off[k];
+
+  // internal ordering
+  s->v_new2old.resize(nv);
+  s->t_new2old.resize(nt);
+  std::iota(s->v_new2old.begin(), s->v_new2old.end(), 0);
+  std::iota(s->t_new2old.begin(), s->t_new2old.end(), 0);
+  if (d->reorder) {
+    float lo[3] = {x0[0], x0[1], x0[2]}, hi[3] = {x0[0], x0[1], x0[2]}, ext[3];
+    for (int i = 0; i < nv; ++i)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], x0[3 * size_t(i) + k]);
+        hi[k] = std::max(hi[k], x0[3 * size_t(i) + k]);
+      }
+    for (int k = 0; k < 3; ++k) ext[k] = hi[k] - lo[k];
+    std::vector<uint32_t> vc(nv), tc(nt);
+    for (int i = 0; i < nv; ++i) vc[i] = morton3(&x0[3 * size_t(i)], lo, ext);
+    std::stable_sort(s->v_new2old.begin(), s->v_new2old.end(),
+                     [&](int a, int b) { return vc[a] < vc[b]; });
+    for (int t = 0; t < nt; ++t) {
+      float c[3] = {0.f, 0.f, 0.f};
+      for (int k = 0; k < 4; ++k)
+        for (int j = 0; j < 3; ++j) c[j] += 0.25f * x0[3 * size_t(d->tets[4 * size_t(t) + k]) + j];
+      tc[t] = morton3(c, lo, ext);
+    }
+    std::stable_sort(s->t_new2old.begin(), s->t_new2old.end(),
+                     [&](int a, int b) { return tc[a] < tc[b]; });
+  }
+  s->v_old2new.resize(nv);
+  for (int i = 0; i < nv; ++i) s->v_old2new[s->v_new2old[i]] = i;
+  s->tets_orig.assign(d->tets, d->tets + 4 * size_t(nt));
+  s->tets_new.resize(4 * size_t(nt));
+  for (int t = 0; t < nt; ++t)
+    for (int k = 0; k < 4; ++k)
+      s->tets_new[4 * size_t(t) + k] = s->v_old2new[d->tets[4 * size_t(s->t_new2old[t]) + k]];
+  std::vector<float> x(3 * size_t(nv));
+  for (int i = 0; i < nv; ++i)
+    for (int k = 0; k < 3; ++k) x[3 * size_t(i) + k] = x0[3 * size_t(s->v_new2old[i]) + k];
+
+  // initialize (neohookean_solver.py:102-107): V0 and inv_Dm per tet
+  std::vector<float4> h_dm0(nt), h_dm1(nt);
+  std::vector<float2> h_dm2(nt);
+  std::vector<int4> h_tet(nt);
+  for (int t = 0; t < nt; ++t) {
+    const int* tv = &s->tets_new[4 * size_t(t)];
+    const float* p[4];
+    for (int k = 0; k < 4; ++k) p[k] = &x[3 * size_t(tv[k])];
+    const float V0 = tet_volume_f32(p[0], p[1], p[2], p[3]);
+    float dm[3][3], inv[3][3];
+    for (int r = 0; r < 3; ++r) {
+      dm[r][0] = p[1][r] - p[0][r];
+      dm[r][1] = p[2][r] - p[0][r];
+      dm[r][2] = p[3][r] - p[0][r];
+    }
+    inv3_f32(dm, inv);
+    h_dm0[t] = make_float4(inv[0][0], inv[0][1], inv[0][2], inv[1][0]);
+    h_dm1[t] = make_float4(inv[1][1], inv[1][2], inv[2][0], inv[2][1]);
+    h_dm2[t] = make_float2(inv[2][2], V0);
+    h_tet[t] = make_int4(tv[0], tv[1], tv[2], tv[3]);
+  }
+
+  // compute_lumped_mass (xpbd_base.py:133-137) with zero velocities
+  std::vector<float> v0(3 * size_t(nv), 0.f);
+  lumped_mass_host(s.get(), x, v0, d->total_mass);
+
+  // valence (cons_num in block mode) and the vertex -> corner CSR
+  std::vector<float> val(nv, 0.f);
+  std::vector<int> off_v(nv + 1, 0);
+  for (int i = 0; i < 4 * nt; ++i) off_v[s->tets_new[i] + 1]++;
+  for (int i = 0; i < nv; ++i) {
+    val[i] = float(off_v[i + 1]);
+    off_v[i + 1] += off_v[i];
+  }
+
+  std::vector<float4> h_pos(nv), h_zero(nv, make_float4(0.f, 0.f, 0.f, 0.f));
+  for (int i = 0; i < nv; ++i)
+    h_pos[i] = make_float4(x[3 * size_t(i)], x[3 * size_t(i) + 1], x[3 * size_t(i) + 2], s->inv_m_host[i]);
+
+  TRY(s->pos.upload(h_pos.data(), nv));
+  TRY(s->xs.upload(h_pos.data(), nv));
+  TRY(s->vel.upload(h_zero.data(), nv));
+  TRY(s->prev.upload(h_zero.data(), nv));
+  TRY(s->xj.upload(h_zero.data(), nv));
+  TRY(s->dacc.upload(h_zero.data(), nv));
+  TRY(s->valence.upload(val.data(), nv));
+  TRY(s->tet.upload(h_tet.data(), nt));
+  TRY(s->dm0.upload(h_dm0.data(), nt));
+  TRY(s->dm1.upload(h_dm1.data(), nt));
+  TRY(s->dm2.upload(h_dm2.data(), nt));
+  TRY(s->lam.alloc(nt));
+  TRY(s->lam.zero());
+  TRY(s->d_old2new.upload(s->v_old2new.data(), nv));
+  TRY(s->scratch_xyz.alloc(3 * size_t(nv)));
+  TRY(s->flag.alloc(1));
+  if (d->accumulate_mode == XPBD_ACCUM_CSR) {
+    std::vector<int> adj(4 * size_t(nt));
+    std::vector<int> fill(off_v.begin(), off_v.end() - 1);
+    for (int t = 0; t < nt; ++t)
+      for (int k = 0; k < 4; ++k) adj[fill[s->tets_new[4 * size_t(t) + k]]++] = 4 * t + k;
+    TRY(s->adj_off.upload(off_v.data(), nv + 1));
+    TRY(s->adj.upload(adj.data(), adj.size()));
+    TRY(s->corner.alloc(4 * size_t(nt)));
+    TRY(s->corner.zero());
+  }
+  TRY(build_maps(s.get()));
+  TRY(s->partial.alloc(3 * size_t(s->ntb)));
+  TRY(s->partial.zero());
+  TRY(s->ticket.alloc(1));
+  TRY(s->ticket.zero());
+  SceneCtrl c0;
+  std::memset(&c0, 0, sizeof(c0));
+  c0.rho_hat = 0.999;  // neohookean_solver.py:65
+  c0.omega = 1.0;
+  TRY(s->ctrl.upload(&c0, 1));
+  TRY(s->logbuf.alloc(6 * size_t(std::max(1, d->constraints_iter))));
+  TRY(s->logbuf.zero());
+  HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->log_host),
+                        sizeof(double) * 6 * std::max(1, d->constraints_iter), 0));
+  TRY(s->table.alloc(1));
+  TRY(build_scene(s.get()));
+  TRY(sync_table(s.get(), s->prm.dt / s->prm.substeps_num, s->prm.damping));
+  HIP_TRY(hipDeviceSynchronize());
+  *out = s.release();
+  return XPBD_OK;
+}
+
+int xpbd_destroy(xpbd_solver* s) {
+  if (!s) return XPBD_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  delete s;
+  return XPBD_OK;
+}
+
+#define CHECK_HANDLE(s)                                               \
+  do {                                                                \
+    if (!(s)) return set_error(XPBD_EINVAL, "solver handle is NULL"); \
+    HIP_TRY(hipSetDevice((s)->device));                               \
+  } while (0)
+
+int xpbd_update_params(xpbd_solver* s, const xpbd_desc* d) {
+  CHECK_HANDLE(s);
+  if (!d) return set_error(XPBD_EINVAL, "desc is NULL");
+  TRY(validate_params(d));
+  if ((d->accumulate_mode == XPBD_ACCUM_CSR) != (s->prm.accumulate_mode == XPBD_ACCUM_CSR))
+    return set_error(XPBD_EINVAL, "accumulate_mode is fixed at creation");
+  if (d->block_size != s->prm.block_size)
+    return set_error(XPBD_EINVAL, "block_size is fixed at creation");
+  const int old_iters = s->prm.constraints_iter;
+  xpbd_desc keep = s->prm;
+  s->prm = *d;
+  s->prm.rest_positions = nullptr;
+  s->prm.tets = nullptr;
+  s->prm.sdfs = nullptr;
+  s->prm.n_sdf = keep.n_sdf;
+  s->prm.reorder = keep.reorder;
+  s->prm.device = keep.device;
+  if (d->constraints_iter > old_iters) {
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    TRY(s->logbuf.alloc(6 * size_t(d->constraints_iter)));
+    TRY(s->logbuf.zero());
+    if (s->log_host) (void)hipHostFree(s->log_host);
+    s->log_host = nullptr;
+    HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->log_host),
+                          sizeof(double) * 6 * d->constraints_iter, 0));
+    s->host_scene.log = s->logbuf.p;
+  }
+  s->table_valid = false;
+  return sync_table(s, s->prm.dt / s->prm.substeps_num, s->prm.damping);
+}
+
+int xpbd_set_sdfs(xpbd_solver* s, const xpbd_sdf* sdfs, int32_t n) {
+  CHECK_HANDLE(s);
+  TRY(validate_sdfs(sdfs, n));
+  s->sdfs.assign(sdfs, sdfs + n);
+  s->prm.n_sdf = n;
+  s->table_valid = false;
+  return sync_table(s, double(s->host_scene.dt), s->prm.damping);
+}
+
+int xpbd_set_gravity(xpbd_solver* s, const double g[3]) {
+  CHECK_HANDLE(s);
+  if (!g) return set_error(XPBD_EINVAL, "gravity is NULL");
+  for (int k = 0; k < 3; ++k) s->prm.gravity[k] = g[k];
+  s->table_valid = false;
+  return XPBD_OK;  // uploaded before the next launch
+}
+
+int xpbd_compute_lumped_mass(xpbd_solver* s, double total_mass) {
+  CHECK_HANDLE(s);
+  std::vector<float> x, v;
+  TRY(download_vec4_xyz(s, s->xs, x));
+  TRY(download_vec4_xyz(s, s->vel, v));
+  s->prm.total_mass = total_mass;
+  lumped_mass_host(s, x, v, total_mass);
+  return upload_inv_m(s);
+}
+
+int xpbd_get_positions(xpbd_solver* s, float* xyz) {
+  CHECK_HANDLE(s);
+  if (!xyz) return set_error(XPBD_EINVAL, "NULL buffer");
+  HIP_TRY(xpbd::launch_gather_xyz(s->scratch_xyz.p, s->xs.p, s->d_old2new.p, s->nv, s->stream));
+  HIP_TRY(hipMemcpyAsync(xyz, s->scratch_xyz.p, sizeof(float) * 3 * s->nv, hipMemcpyDeviceToHost,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_get_positions_device(xpbd_solver* s, void* dev_xyz) {
+  CHECK_HANDLE(s);
+  if (!dev_xyz) return set_error(XPBD_EINVAL, "NULL buffer");
+  HIP_TRY(xpbd::launch_gather_xyz(static_cast<float*>(dev_xyz), s->xs.p, s->d_old2new.p, s->nv,
+                                  s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_set_positions(xpbd_solver* s, const float* xyz) {
+  CHECK_HANDLE(s);
+  if (!xyz) return set_error(XPBD_EINVAL, "NULL buffer");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(s->scratch_xyz.p, xyz, sizeof(float) * 3 * s->nv, hipMemcpyHostToDevice));
+  return set_positions_common(s, s->scratch_xyz.p);
+}
+
+int xpbd_set_positions_device(xpbd_solver* s, const void* dev_xyz) {
+  CHECK_HANDLE(s);
+  if (!dev_xyz) return set_error(XPBD_EINVAL, "NULL buffer");
+  return set_positions_common(s, static_cast<const float*>(dev_xyz));
+}
+
+int xpbd_get_velocities(xpbd_solver* s, float* xyz) {
+  CHECK_HANDLE(s);
+  if (!xyz) return set_error(XPBD_EINVAL, "NULL buffer");
+  HIP_TRY(xpbd::launch_gather_xyz(s->scratch_xyz.p, s->vel.p, s->d_old2new.p, s->nv, s->stream));
+  HIP_TRY(hipMemcpyAsync(xyz, s->scratch_xyz.p, sizeof(float) * 3 * s->nv, hipMemcpyDeviceToHost,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_set_velocities(xpbd_solver* s, const float* xyz) {
+  CHECK_HANDLE(s);
+  if (!xyz) return set_error(XPBD_EINVAL, "NULL buffer");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(s->scratch_xyz.p, xyz, sizeof(float) * 3 * s->nv, hipMemcpyHostToDevice));
+  HIP_TRY(xpbd::launch_scatter_xyz(s->vel.p, s->scratch_xyz.p, s->d_old2new.p, s->nv, 0, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_get_predicted(xpbd_solver* s, float* xyz) {
+  CHECK_HANDLE(s);
+  if (!xyz) return set_error(XPBD_EINVAL, "NULL buffer");
+  HIP_TRY(xpbd::launch_gather_xyz(s->scratch_xyz.p, s->pos.p, s->d_old2new.p, s->nv, s->stream));
+  HIP_TRY(hipMemcpyAsync(xyz, s->scratch_xyz.p, sizeof(float) * 3 * s->nv, hipMemcpyDeviceToHost,
+                         s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_get_masses(xpbd_solver* s, float* m, float* inv_m) {
+  CHECK_HANDLE(s);
+  for (int i = 0; i < s->nv; ++i) {
+    const int j = s->v_old2new[i];
+    if (m) m[i] = s->m_host[j];
+    if (inv_m) inv_m[i] = s->inv_m_host[j];
+  }
+  return XPBD_OK;
+}
+
+int xpbd_get_multipliers(xpbd_solver* s, float* lambda_d, float* lambda_h) {
+  CHECK_HANDLE(s);
+  std::vector<float2> l(s->nt);
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(l.data(), s->lam.p, sizeof(float2) * s->nt, hipMemcpyDeviceToHost));
+  for (int t = 0; t < s->nt; ++t) {
+    const int o = s->t_new2old[t];
+    if (lambda_d) lambda_d[o] = l[t].x;
+    if (lambda_h) lambda_h[o] = l[t].y;
+  }
+  return XPBD_OK;
+}
+
+int xpbd_get_visual_indices(xpbd_solver* s, uint32_t* out) {
+  if (!s || !out) return set_error(XPBD_EINVAL, "NULL argument");
+  // initialize_visual_indices (xpbd_base.py:78-87)
+  static const int ind[4][3] = {{0, 2, 1}, {0, 3, 2}, {0, 1, 3}, {1, 2, 3}};
+  for (int c = 0; c < s->nt; ++c)
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 3; ++j)
+        out[(size_t(c) * 4 + i) * 3 + j] = uint32_t(s->tets_orig[4 * size_t(c) + ind[i][j]]);
+  return XPBD_OK;
+}
+
+int xpbd_apply_external_forces(xpbd_solver* s, double dt) {
+  CHECK_HANDLE(s);
+  TRY(sync_table(s, dt, s->prm.damping));
+  HIP_TRY(xpbd::launch_predict(s->table.p, s->vmap.p, s->nvb, s->bs_v, 1, 0, 0, s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_solve_sdf_collision(xpbd_solver* s, double dt) {
+  CHECK_HANDLE(s);
+  TRY(sync_table(s, dt, s->prm.damping));
+  HIP_TRY(xpbd::launch_predict(s->table.p, s->vmap.p, s->nvb, s->bs_v, 0, 1, 0, s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_solve_step(xpbd_solver* s, double dt) {
+  CHECK_HANDLE(s);
+  TRY(sync_table(s, dt, s->prm.damping));
+  TRY(enqueue_solve_step(s, false));
+  return fetch_log(s);
+}
+
+int xpbd_advance(xpbd_solver* s, double dt, double damping) {
+  CHECK_HANDLE(s);
+  TRY(sync_table(s, dt, damping));
+  HIP_TRY(xpbd::launch_advance(s->table.p, s->vmap.p, s->nvb, s->bs_v, s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_substep(xpbd_solver* s, double sub_dt) {
+  CHECK_HANDLE(s);
+  return run_substep(s, sub_dt);
+}
+
+int xpbd_frame(xpbd_solver* s, int32_t n_frames) {
+  CHECK_HANDLE(s);
+  for (int f = 0; f < n_frames; ++f) {
+    // solve (neohookean_solver.py:562-587)
+    double frame_time_left = s->prm.frame_dt;
+    s->substep = 0;
+    while (frame_time_left > 0.0) {
+      const double dt0 = std::min(s->prm.dt, frame_time_left);
+      frame_time_left -= dt0;
+      const double sub_dt = dt0 / s->prm.substeps_num;
+      for (int k = 0; k < s->prm.substeps_num; ++k) {
+        s->substep = k;
+        TRY(run_substep(s, sub_dt));
+      }
+      s->time += dt0;
+    }
+    s->frame += 1;
+  }
+  return XPBD_OK;
+}
+
+int xpbd_synchronize(xpbd_solver* s) {
+  CHECK_HANDLE(s);
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return XPBD_OK;
+}
+
+int xpbd_get_stream(xpbd_solver* s, void** stream) {
+  if (!s || !stream) return set_error(XPBD_EINVAL, "NULL argument");
+  *stream = static_cast<void*>(s->stream);
+  return XPBD_OK;
+}
+
+int xpbd_set_logging(xpbd_solver* s, int32_t enabled) {
+  if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
+  s->logging = enabled != 0;
+  return XPBD_OK;
+}
+
+int xpbd_log_size(xpbd_solver* s, int64_t* rows) {
+  if (!s || !rows) return set_error(XPBD_EINVAL, "NULL argument");
+  *rows = int64_t(s->log_rows.size() / 8);
+  return XPBD_OK;
+}
+
+int xpbd_get_log(xpbd_solver* s, double* rows, int64_t max_rows) {
+  if (!s || !rows) return set_error(XPBD_EINVAL, "NULL argument");
+  const size_t n = std::min<size_t>(size_t(std::max<int64_t>(max_rows, 0)), s->log_rows.size() / 8);
+  std::memcpy(rows, s->log_rows.data(), n * 8 * sizeof(double));
+  return XPBD_OK;
+}
+
+int xpbd_clear_log(xpbd_solver* s) {
+  if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
+  s->log_rows.clear();
+  return XPBD_OK;
+}
+
+int xpbd_get_state(xpbd_solver* s, int32_t* frame, int32_t* substep, double* time,
+                   double* rho_hat) {
+  CHECK_HANDLE(s);
+  if (frame) *frame = s->frame;
+  if (substep) *substep = s->substep;
+  if (time) *time = s->time;
+  if (rho_hat) {
+    SceneCtrl c;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(&c, s->ctrl.p, sizeof(c), hipMemcpyDeviceToHost));
+    *rho_hat = c.rho_hat;
+  }
+  return XPBD_OK;
+}
+
+int xpbd_set_state(xpbd_solver* s, int32_t frame, double time, double rho_hat) {
+  CHECK_HANDLE(s);
+  SceneCtrl c;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(&c, s->ctrl.p, sizeof(c), hipMemcpyDeviceToHost));
+  if (!std::isnan(rho_hat)) {
+    c.rho_hat = rho_hat;
+    HIP_TRY(hipMemcpy(s->ctrl.p, &c, sizeof(c), hipMemcpyHostToDevice));
+  }
+  s->frame = frame;
+  s->time = time;
+  return XPBD_OK;
+}
+
+int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets) {
+  if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
+  if (n_verts) *n_verts = s->nv;
+  if (n_tets) *n_tets = s->nt;
+  return XPBD_OK;
+}
+
+int xpbd_set_exec_path(xpbd_solver* s, int32_t path) {
+  if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
+  if (path != XPBD_PATH_KERNELS && path != XPBD_PATH_GRAPH)
+    return set_error(XPBD_EINVAL, "unknown exec path %d", path);
+  s->path = path;
+  return XPBD_OK;
+}
+
+int xpbd_set_kernel_timing(xpbd_solver* s, int32_t enabled) {
+  CHECK_HANDLE(s);
+  TRY(events_collect(s));
+  s->timing = enabled != 0;
+  s->timed_ms = 0.0;
+  s->timed_launches = 0;
+  return XPBD_OK;
+}
+
+int xpbd_get_kernel_time(xpbd_solver* s, double* total_ms, int64_t* launches) {
+  CHECK_HANDLE(s);
+  TRY(events_collect(s));
+  if (total_ms) *total_ms = s->timed_ms;
+  if (launches) *launches = s->timed_launches;
+  return XPBD_OK;
+}
+
+int xpbd_check_finite(xpbd_solver* s) {
+  CHECK_HANDLE(s);
+  int zero = 0, hit = 0;
+  HIP_TRY(hipMemcpyAsync(s->flag.p, &zero, sizeof(int), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(xpbd::launch_check_finite(s->xs.p, s->nv, s->flag.p, s->stream));
+  HIP_TRY(xpbd::launch_check_finite(s->pos.p, s->nv, s->flag.p, s->stream));
+  HIP_TRY(hipMemcpyAsync(&hit, s->flag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (hit) return set_error(XPBD_ENUMERIC, "non-finite vertex position in the solver state");
+  return XPBD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,29 @@
+// Host-callable launch wrappers of the gfx950 kernels in xpbd_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace xpbd {
+
+struct SceneDev;
+struct WorkItem;
+
+hipError_t launch_predict(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                          int external, int collide, int save_prev, hipStream_t st);
+hipError_t launch_project(const SceneDev* scenes, const WorkItem* tmap, int nblocks, int bs,
+                          int first, hipStream_t st);
+hipError_t launch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                          hipStream_t st);
+hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nblocks, int bs,
+                           int it, hipStream_t st);
+hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                        hipStream_t st);
+hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                          hipStream_t st);
+hipError_t launch_scatter_xyz(float4* dst, const float* src, const int* old_to_new, int n,
+                              int keep_w, hipStream_t st);
+hipError_t launch_gather_xyz(float* dst, const float4* src, const int* old_to_new, int n,
+                             hipStream_t st);
+hipError_t launch_check_finite(const float4* pos, int n, int* flag, hipStream_t st);
+
+}  // namespace xpbd

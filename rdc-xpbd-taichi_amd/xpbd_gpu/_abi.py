@@ -1,0 +1,195 @@
+"""ctypes binding of libxpbd_hip.so (the C ABI declared in include/xpbd_abi.h).
+
+The library is required: importing the solver without it raises, there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+LIB_NAME = "libxpbd_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+XPBD_OK = 0
+XPBD_EINVAL = -1
+XPBD_EHIP = -2
+XPBD_ENOMEM = -3
+XPBD_ESTATE = -4
+XPBD_ENUMERIC = -5
+XPBD_MAX_SDF = 8
+XPBD_ABI_VERSION = 1
+
+ACCUM_ATOMIC = 0
+ACCUM_CSR = 1
+SDF_PLANE = 0
+SDF_SPHERE = 1
+PATH_KERNELS = 0
+PATH_GRAPH = 1
+
+# Every symbol the header declares; tests check that the library exports them.
+EXPORTED = (
+    "xpbd_abi_info", "xpbd_desc_init", "xpbd_create", "xpbd_destroy", "xpbd_update_params",
+    "xpbd_set_sdfs", "xpbd_set_gravity", "xpbd_compute_lumped_mass",
+    "xpbd_get_positions", "xpbd_set_positions", "xpbd_get_positions_device",
+    "xpbd_set_positions_device", "xpbd_get_velocities", "xpbd_set_velocities",
+    "xpbd_get_predicted", "xpbd_get_masses", "xpbd_get_multipliers",
+    "xpbd_get_visual_indices", "xpbd_apply_external_forces", "xpbd_solve_sdf_collision",
+    "xpbd_solve_step", "xpbd_advance", "xpbd_substep", "xpbd_frame", "xpbd_synchronize",
+    "xpbd_get_stream", "xpbd_set_logging", "xpbd_log_size", "xpbd_get_log", "xpbd_clear_log",
+    "xpbd_get_state", "xpbd_set_state", "xpbd_get_sizes", "xpbd_set_exec_path",
+    "xpbd_set_kernel_timing", "xpbd_get_kernel_time", "xpbd_check_finite", "xpbd_last_error",
+)
+
+
+class XpbdSdf(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("fixed", ctypes.c_int32),
+        ("p", ctypes.c_double * 4),
+        ("dir", ctypes.c_double * 3),
+    ]
+
+
+class XpbdDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_verts", ctypes.c_int32),
+        ("n_tets", ctypes.c_int32),
+        ("rest_positions", ctypes.POINTER(ctypes.c_float)),
+        ("tets", ctypes.POINTER(ctypes.c_int32)),
+        ("young_modulus", ctypes.c_double),
+        ("poisson_ratio", ctypes.c_double),
+        ("scale", ctypes.c_double),
+        ("offset", ctypes.c_double * 3),
+        ("frame_dt", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("substeps_num", ctypes.c_int32),
+        ("constraints_iter", ctypes.c_int32),
+        ("hydrostatic_relaxation", ctypes.c_double),
+        ("deviatoric_relaxation", ctypes.c_double),
+        ("cheb_enable", ctypes.c_int32),
+        ("cheb_warmup", ctypes.c_int32),
+        ("cheb_gamma", ctypes.c_double),
+        ("block_neohookean_enable", ctypes.c_int32),
+        ("monitor_convergence", ctypes.c_int32),
+        ("damping", ctypes.c_double),
+        ("gravity", ctypes.c_double * 3),
+        ("total_mass", ctypes.c_double),
+        ("n_sdf", ctypes.c_int32),
+        ("sdfs", ctypes.POINTER(XpbdSdf)),
+        ("accumulate_mode", ctypes.c_int32),
+        ("reorder", ctypes.c_int32),
+        ("block_size", ctypes.c_int32),
+        ("device", ctypes.c_int32),
+    ]
+
+
+class XpbdError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"xpbd error {code}: {message}")
+        self.code = code
+
+
+_lib = None
+
+
+def library_available() -> bool:
+    return os.path.exists(LIB_PATH)
+
+
+def load() -> ctypes.CDLL:
+    """Loads libxpbd_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python rdc-xpbd-taichi_amd/build.py` "
+            "(the solver has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    pp = ctypes.POINTER(ctypes.c_void_p)
+    fp = ctypes.POINTER(ctypes.c_float)
+    dp = ctypes.POINTER(ctypes.c_double)
+    i32 = ctypes.c_int32
+    sigs = {
+        "xpbd_abi_info": [ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_size_t),
+                          ctypes.POINTER(ctypes.c_size_t)],
+        "xpbd_desc_init": [ctypes.POINTER(XpbdDesc)],
+        "xpbd_create": [ctypes.POINTER(XpbdDesc), pp],
+        "xpbd_destroy": [vp],
+        "xpbd_update_params": [vp, ctypes.POINTER(XpbdDesc)],
+        "xpbd_set_sdfs": [vp, ctypes.POINTER(XpbdSdf), i32],
+        "xpbd_set_gravity": [vp, dp],
+        "xpbd_compute_lumped_mass": [vp, ctypes.c_double],
+        "xpbd_get_positions": [vp, fp],
+        "xpbd_set_positions": [vp, fp],
+        "xpbd_get_positions_device": [vp, vp],
+        "xpbd_set_positions_device": [vp, vp],
+        "xpbd_get_velocities": [vp, fp],
+        "xpbd_set_velocities": [vp, fp],
+        "xpbd_get_predicted": [vp, fp],
+        "xpbd_get_masses": [vp, fp, fp],
+        "xpbd_get_multipliers": [vp, fp, fp],
+        "xpbd_get_visual_indices": [vp, ctypes.POINTER(ctypes.c_uint32)],
+        "xpbd_apply_external_forces": [vp, ctypes.c_double],
+        "xpbd_solve_sdf_collision": [vp, ctypes.c_double],
+        "xpbd_solve_step": [vp, ctypes.c_double],
+        "xpbd_advance": [vp, ctypes.c_double, ctypes.c_double],
+        "xpbd_substep": [vp, ctypes.c_double],
+        "xpbd_frame": [vp, i32],
+        "xpbd_synchronize": [vp],
+        "xpbd_get_stream": [vp, pp],
+        "xpbd_set_logging": [vp, i32],
+        "xpbd_log_size": [vp, ctypes.POINTER(ctypes.c_int64)],
+        "xpbd_get_log": [vp, dp, ctypes.c_int64],
+        "xpbd_clear_log": [vp],
+        "xpbd_get_state": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                           dp, dp],
+        "xpbd_set_state": [vp, i32, ctypes.c_double, ctypes.c_double],
+        "xpbd_get_sizes": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
+        "xpbd_set_exec_path": [vp, i32],
+        "xpbd_set_kernel_timing": [vp, i32],
+        "xpbd_get_kernel_time": [vp, dp, ctypes.POINTER(ctypes.c_int64)],
+        "xpbd_check_finite": [vp],
+        "xpbd_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+    }
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    ver = ctypes.c_int32()
+    dsz = ctypes.c_size_t()
+    ssz = ctypes.c_size_t()
+    lib.xpbd_abi_info(ctypes.byref(ver), ctypes.byref(dsz), ctypes.byref(ssz))
+    if (ver.value != XPBD_ABI_VERSION or dsz.value != ctypes.sizeof(XpbdDesc)
+            or ssz.value != ctypes.sizeof(XpbdSdf)):
+        raise ImportError(
+            f"{LIB_PATH}: ABI mismatch (version {ver.value}, desc {dsz.value} B vs "
+            f"{ctypes.sizeof(XpbdDesc)} B, sdf {ssz.value} B vs {ctypes.sizeof(XpbdSdf)} B)")
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    buf = ctypes.create_string_buffer(1024)
+    load().xpbd_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(rc: int):
+    if rc != XPBD_OK:
+        msg = last_error()
+        if rc == XPBD_EINVAL:
+            raise ValueError(msg)
+        raise XpbdError(rc, msg)
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

@@ -1,0 +1,120 @@
+"""Signed-distance colliders with the reference's Python surface.
+
+Mirrors `xpbd_gpu/sdf.py` of the reference: `SdfModel(fixed)` (:6-27),
+`PlaneSdf(point, dir, fixed)` (:59-74) and `SphereSdf(center, radius, fixed)`
+(:77-95).  The distance/normal/check functions run inside the gfx950 kernels;
+these objects only carry the parameters, exposed as `field[None]` like the
+reference's Taichi scalar fields (`usd_export.py:76-87` reads them that way).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _abi
+
+
+class _Vec(np.ndarray):
+    """A small float32 vector that also answers `.to_numpy()` like a ti.Vector."""
+
+    def to_numpy(self):
+        return np.array(self, dtype=np.float32)
+
+
+def _vec(v) -> _Vec:
+    return np.asarray(v, dtype=np.float32).reshape(3).copy().view(_Vec)
+
+
+class _Scalar:
+    """Stand-in for a 0-d Taichi field: `f[None]` reads, `f[None] = v` writes."""
+
+    def __init__(self, value, on_change=None):
+        self._value = value
+        self._on_change = on_change
+
+    def __getitem__(self, key):
+        if key is not None and key != ():
+            raise KeyError(key)
+        return self._value
+
+    def __setitem__(self, key, value):
+        if key is not None and key != ():
+            raise KeyError(key)
+        if isinstance(self._value, np.ndarray):
+            value = _vec(value)
+        self._value = value
+        if self._on_change is not None:
+            self._on_change()
+
+
+class SdfModel:
+    """Base collider (sdf.py:6-27).  The obstacle velocity is always zero."""
+
+    kind = None
+
+    def __init__(self, fixed):
+        self.vel = _Scalar(_vec((0.0, 0.0, 0.0)))
+        self.fixed = _Scalar(int(bool(fixed)))
+
+    def to_abi(self) -> _abi.XpbdSdf:
+        raise NotImplementedError
+
+
+class PlaneSdf(SdfModel):
+    """phi(p) = (p - point) . dir with dir normalized at construction (sdf.py:59-74)."""
+
+    kind = _abi.SDF_PLANE
+
+    def __init__(self, point, dir, fixed=False):
+        super().__init__(fixed=fixed)
+        d = np.asarray(dir, dtype=np.float64).reshape(3)
+        n = np.linalg.norm(d)
+        if n == 0:
+            raise ValueError("PlaneSdf dir must be non-zero")
+        self.point = _Scalar(_vec(point))
+        self.dir = _Scalar(_vec(d / n))
+        self._dir64 = d / n
+
+    def to_abi(self) -> _abi.XpbdSdf:
+        s = _abi.XpbdSdf()
+        s.kind = self.kind
+        s.fixed = int(bool(self.fixed[None]))
+        pt = np.asarray(self.point[None], dtype=np.float64)
+        for k in range(3):
+            s.p[k] = float(pt[k])
+            s.dir[k] = float(np.asarray(self.dir[None], dtype=np.float64)[k])
+        return s
+
+
+class SphereSdf(SdfModel):
+    """phi(p) = |p - center| - radius (sdf.py:77-95)."""
+
+    kind = _abi.SDF_SPHERE
+
+    def __init__(self, center, radius, fixed=False):
+        super().__init__(fixed=fixed)
+        self.center = _Scalar(_vec(center))
+        self.radius = _Scalar(float(radius))
+
+    def to_abi(self) -> _abi.XpbdSdf:
+        s = _abi.XpbdSdf()
+        s.kind = self.kind
+        s.fixed = int(bool(self.fixed[None]))
+        c = np.asarray(self.center[None], dtype=np.float64)
+        for k in range(3):
+            s.p[k] = float(c[k])
+        s.p[3] = float(self.radius[None])
+        return s
+
+
+def as_sdf_list(sdf):
+    """The reference's SDF argument handling (xpbd_base.py:61-67)."""
+    if isinstance(sdf, (list, tuple)):
+        sdfs = [s for s in sdf if s is not None]
+    else:
+        sdfs = [sdf] if sdf is not None else []
+    if len(sdfs) == 0:
+        raise ValueError("At least one SDF object must be provided.")
+    for s in sdfs:
+        if not isinstance(s, SdfModel):
+            raise TypeError(f"unsupported SDF type {type(s).__name__}")
+    return sdfs

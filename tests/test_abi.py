@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every symbol include/xpbd_abi.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from xpbd_gpu import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "xpbd_abi.h")
+
+
+def declared_symbols():
+    text = open(HEADER, encoding="utf-8").read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\bint\s+(xpbd_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_symbols() == sorted(_abi.EXPORTED)
+
+
+@pytest.mark.skipif(not _abi.library_available(), reason="libxpbd_hip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_abi.LIB_PATH)
+    for name in declared_symbols():
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.skipif(not _abi.library_available(), reason="libxpbd_hip.so not built")
+def test_struct_layout_matches():
+    lib = _abi.load()  # raises on a layout mismatch
+    d = _abi.XpbdDesc()
+    assert lib.xpbd_desc_init(ctypes.byref(d)) == 0
+    assert d.young_modulus == 200000.0 and d.poisson_ratio == 0.495
+    assert d.substeps_num == 20 and d.constraints_iter == 5
+    assert d.cheb_gamma == 0.6666 and d.cheb_warmup == 5 and d.cheb_enable == 1
+    assert d.block_neohookean_enable == 0 and d.gravity[1] == -9.8 and d.damping == 5.0
+
+
+@pytest.mark.skipif(not _abi.library_available(), reason="libxpbd_hip.so not built")
+def test_errors_are_reported_without_gpu_work():
+    lib = _abi.load()
+    d = _abi.XpbdDesc()
+    lib.xpbd_desc_init(ctypes.byref(d))
+    h = ctypes.c_void_p()
+    assert lib.xpbd_create(ctypes.byref(d), ctypes.byref(h)) == _abi.XPBD_EINVAL
+    assert "mesh" in _abi.last_error()
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_abi, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(_abi, "_lib", None)
+    with pytest.raises(ImportError):
+        _abi.load()
