@@ -42,6 +42,10 @@ extern "C" {
 #define XPBD_ACCUM_ATOMIC 0 /* float atomics into per-vertex accumulators */
 #define XPBD_ACCUM_CSR 1    /* deterministic vertex-side gather, fixed order */
 
+/* layout: how tets are laid out and processed on the device */
+#define XPBD_LAYOUT_FLAT 0  /* one thread per tet, inv_Dm streamed from HBM */
+#define XPBD_LAYOUT_PATCH 1 /* one workgroup per patch of <= 256 tets staged in LDS */
+
 /* kind of an SDF collider (xpbd_gpu/sdf.py) */
 #define XPBD_SDF_PLANE 0  /* PlaneSdf(point, dir, fixed), sdf.py:59-74 */
 #define XPBD_SDF_SPHERE 1 /* SphereSdf(center, radius, fixed), sdf.py:77-95 */
@@ -83,15 +87,16 @@ typedef struct xpbd_desc {
   /* execution options (no reference counterpart) */
   int32_t accumulate_mode;     /* XPBD_ACCUM_ATOMIC or XPBD_ACCUM_CSR */
   int32_t reorder;             /* 1: Morton-reorder tets and vertices internally */
-  int32_t block_size;          /* threads per workgroup for tet kernels (64..1024) */
+  int32_t block_size;          /* threads per workgroup for flat tet kernels (64..1024) */
   int32_t device;              /* HIP device ordinal */
+  int32_t layout;              /* XPBD_LAYOUT_PATCH (default) or XPBD_LAYOUT_FLAT */
 } xpbd_desc;
 
 typedef struct xpbd_solver xpbd_solver;
 
 /* Version of this ABI and the sizes of its structs as compiled, so that a
  * binding can check its own struct layout. */
-#define XPBD_ABI_VERSION 1
+#define XPBD_ABI_VERSION 2
 int xpbd_abi_info(int32_t* version, size_t* desc_size, size_t* sdf_size);
 
 /* Fills *d with the reference defaults (everything but the mesh). */
@@ -186,6 +191,12 @@ int xpbd_get_kernel_time(xpbd_solver* s, double* total_ms, int64_t* launches);
 /* Reports XPBD_ENUMERIC when any vertex position (x or pred_x) is NaN or
  * infinite; the reference never checks (xpbd_base.py:139-160). */
 int xpbd_check_finite(xpbd_solver* s);
+
+/* Host-only description of the patch layout a mesh would get (no device
+ * work): number of patches, number of (patch, vertex) slots, and the largest
+ * vertex count of a patch.  tets: n_tets*4 ids in the order they are given. */
+int xpbd_patch_stats(const int32_t* tets, int32_t n_tets, int32_t n_verts, int32_t* n_patches,
+                     int64_t* n_slots, int32_t* max_patch_verts, int32_t* max_patch_tets);
 
 /* Copies the last error message of the calling thread into buf. */
 int xpbd_last_error(char* buf, size_t len);

@@ -47,13 +47,16 @@ struct SceneCtrl {
 };
 
 struct SceneDev {
+  // vertex streams (n_verts)
   float4* pos;
   float4* xs;
   float4* vel;
   float4* prev;
   float4* xj;
   float4* dacc;
+  const float4* rest;  // rest positions (after scale/offset), w unused
   const float* valence;
+  // flat layout: one thread per tet
   const int* adj_off;
   const int* adj;
   float4* corner;
@@ -61,15 +64,26 @@ struct SceneDev {
   const float4* dm0;
   const float4* dm1;
   const float2* dm2;
+  // patch layout: one workgroup per patch of <= 256 tets / <= 256 vertices
+  const int* p_tet_begin;   // n_patches + 1
+  const int* p_vert_begin;  // n_patches + 1; also the slot ids of the patch vertices
+  const int* p_vert_ids;    // global vertex id of each slot
+  const unsigned* tet_local;  // 4 x u8 local vertex ids per tet
+  const int* vslot_off;     // n_verts + 1: slots of each vertex, ascending patch
+  const int* vslot;
+  float4* slot_acc;         // per-slot partial delta_x (xyz) and constraint count (w)
+  const int* lc_off;        // per slot: its corners in the patch, ascending tet
+  const unsigned short* lc; // corner = local tet * 4 + k
   float2* lam;
-  double* partial;  // 3 doubles per tet block: sum r, sum |C_H|, sum |C_D|
-  unsigned* ticket;
+  double* partial;  // 3 doubles per tet workgroup / patch: sum r, sum |C_H|, sum |C_D|
   SceneCtrl* ctrl;
   double* log;      // 6 doubles per iteration: iter, r_hat, rho_hat, omega, hydro, dev
   int n_verts;
   int n_tets;
-  int n_tet_blocks;
-  int tet_block;    // threads per tet block (partial index = first_tet / tet_block)
+  int n_partials;
+  int tet_block;    // flat layout: threads per tet workgroup
+  int n_patches;
+  int layout;       // 0 flat, 1 patch
   // parameters as the reference kernels see them (f32 constants / arguments)
   float lame1, lame2;
   float dt;
@@ -248,6 +262,30 @@ __device__ __forceinline__ float3 sdf_project(const SdfDev& s, float3 x, float3 
     }
   }
   return x;
+}
+
+// V0 and inv_Dm of a tet from its rest positions, evaluated exactly as the
+// set-up does (calc_tet_volume, xpbd_base.py:6-8; Matrix.inverse of
+// cols(x1-x0, x2-x0, x3-x0), neohookean_solver.py:102-107) with no
+// contraction, so every bit matches the values the flat layout stores.
+__device__ __forceinline__ void rest_shape(float3 p0, float3 p1, float3 p2, float3 p3, Mat3& inv,
+                                           float& V0) {
+#pragma clang fp contract(off)
+  const float3 a = p1 - p0, b = p2 - p0, c = p3 - p0;
+  const float3 ab = make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+  V0 = (ab.x * c.x + ab.y * c.y + ab.z * c.z) / 6.f;
+  const float m[3][3] = {{a.x, b.x, c.x}, {a.y, b.y, c.y}, {a.z, b.z, c.z}};
+  const float det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
+                    m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
+                    m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
+  const float inv_det = 1.f / det;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      inv.m[j][i] = inv_det * (m[i1][j1] * m[i2][j2] - m[i2][j1] * m[i1][j2]);
+    }
 }
 
 }  // namespace xpbd

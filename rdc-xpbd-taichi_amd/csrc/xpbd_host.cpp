@@ -168,6 +168,80 @@ int validate_sdfs(const xpbd_sdf* sdfs, int n) {
   return XPBD_OK;
 }
 
+// Patches: consecutive (Morton-ordered) tets grouped into workgroup-sized
+// pieces of at most kPatchTets tets touching at most kPatchVerts vertices, so
+// that local vertex ids fit in a byte.  Each (patch, vertex) pair is a slot;
+// a vertex's slots are listed in ascending patch order.
+constexpr int kPatchTets = 256;
+constexpr int kPatchVerts = 256;
+
+struct Patches {
+  std::vector<int> tet_begin, vert_begin, vert_ids, vslot_off, vslot, lc_off;
+  std::vector<unsigned> tet_local;
+  std::vector<unsigned short> lc;
+};
+
+void build_patches(const std::vector<int>& tets, int nv, int nt, Patches& P) {
+  P.tet_begin.assign(1, 0);
+  P.vert_begin.assign(1, 0);
+  P.vert_ids.clear();
+  P.tet_local.assign(nt, 0u);
+  P.lc_off.assign(1, 0);
+  P.lc.clear();
+  std::vector<int> stamp(nv, -1), local(nv, 0);
+  std::vector<int> cur;
+  int start = 0;
+  auto close = [&](int end) {
+    const int p = int(P.tet_begin.size()) - 1;
+    std::sort(cur.begin(), cur.end());
+    for (size_t i = 0; i < cur.size(); ++i) local[cur[i]] = int(i);
+    std::vector<std::vector<unsigned short>> corners(cur.size());
+    for (int t = start; t < end; ++t) {
+      unsigned packed = 0;
+      for (int k = 0; k < 4; ++k) {
+        const int l = local[tets[4 * size_t(t) + k]];
+        packed |= unsigned(l) << (8 * k);
+        corners[l].push_back((unsigned short)(4 * (t - start) + k));
+      }
+      P.tet_local[t] = packed;
+    }
+    for (size_t i = 0; i < cur.size(); ++i) {
+      P.vert_ids.push_back(cur[i]);
+      P.lc.insert(P.lc.end(), corners[i].begin(), corners[i].end());
+      P.lc_off.push_back(int(P.lc.size()));
+    }
+    P.tet_begin.push_back(end);
+    P.vert_begin.push_back(int(P.vert_ids.size()));
+    (void)p;
+    cur.clear();
+    start = end;
+  };
+  for (int t = 0; t < nt; ++t) {
+    const int p = int(P.tet_begin.size()) - 1;
+    int fresh = 0;
+    for (int k = 0; k < 4; ++k) fresh += stamp[tets[4 * size_t(t) + k]] != p;
+    if (t - start == kPatchTets || int(cur.size()) + fresh > kPatchVerts) {
+      close(t);
+    }
+    const int q = int(P.tet_begin.size()) - 1;
+    for (int k = 0; k < 4; ++k) {
+      const int g = tets[4 * size_t(t) + k];
+      if (stamp[g] != q) {
+        stamp[g] = q;
+        cur.push_back(g);
+      }
+    }
+  }
+  if (nt > start) close(nt);
+  // vertex -> slots, ascending slot (= ascending patch)
+  P.vslot_off.assign(nv + 1, 0);
+  for (int g : P.vert_ids) P.vslot_off[g + 1]++;
+  for (int v = 0; v < nv; ++v) P.vslot_off[v + 1] += P.vslot_off[v];
+  P.vslot.assign(P.vert_ids.size(), 0);
+  std::vector<int> fill(P.vslot_off.begin(), P.vslot_off.end() - 1);
+  for (size_t sl = 0; sl < P.vert_ids.size(); ++sl) P.vslot[fill[P.vert_ids[sl]]++] = int(sl);
+}
+
 }  // namespace
 
 struct xpbd_solver {
@@ -182,14 +256,19 @@ struct xpbd_solver {
   std::vector<int> tets_new;       // internal order, internal vertex ids
   std::vector<float> m_host, inv_m_host;  // internal order
 
-  DevBuf<float4> pos, xs, vel, prev, xj, dacc, corner;
+  DevBuf<float4> pos, xs, vel, prev, xj, dacc, corner, rest, slot_acc;
+  DevBuf<int> p_tet_begin, p_vert_begin, p_vert_ids, vslot_off, vslot, lc_off;
+  DevBuf<unsigned> tet_local;
+  DevBuf<unsigned short> lc;
+  DevBuf<WorkItem> pmap;
+  int n_patches = 0;
+  int layout = XPBD_LAYOUT_PATCH;
   DevBuf<float> valence;
   DevBuf<int> adj_off, adj, d_old2new;
   DevBuf<int4> tet;
   DevBuf<float4> dm0, dm1;
   DevBuf<float2> dm2, lam;
   DevBuf<double> partial, logbuf;
-  DevBuf<unsigned> ticket;
   DevBuf<SceneCtrl> ctrl;
   DevBuf<SceneDev> table;
   DevBuf<WorkItem> vmap, tmap;
@@ -240,6 +319,8 @@ int validate_params(const xpbd_desc* d) {
   if (d->accumulate_mode != XPBD_ACCUM_ATOMIC && d->accumulate_mode != XPBD_ACCUM_CSR)
     return set_error(XPBD_EINVAL, "unknown accumulate_mode %d", d->accumulate_mode);
   if (!(d->frame_dt > 0) || !(d->dt > 0)) return set_error(XPBD_EINVAL, "frame_dt and dt must be > 0");
+  if (d->layout != XPBD_LAYOUT_FLAT && d->layout != XPBD_LAYOUT_PATCH)
+    return set_error(XPBD_EINVAL, "unknown layout %d", d->layout);
   return XPBD_OK;
 }
 
@@ -285,14 +366,25 @@ int build_scene(xpbd_solver* s) {
   S.dm0 = s->dm0.p;
   S.dm1 = s->dm1.p;
   S.dm2 = s->dm2.p;
+  S.rest = s->rest.p;
+  S.p_tet_begin = s->p_tet_begin.p;
+  S.p_vert_begin = s->p_vert_begin.p;
+  S.p_vert_ids = s->p_vert_ids.p;
+  S.tet_local = s->tet_local.p;
+  S.vslot_off = s->vslot_off.p;
+  S.vslot = s->vslot.p;
+  S.slot_acc = s->slot_acc.p;
+  S.lc_off = s->lc_off.p;
+  S.lc = s->lc.p;
   S.lam = s->lam.p;
   S.partial = s->partial.p;
-  S.ticket = s->ticket.p;
   S.ctrl = s->ctrl.p;
   S.log = s->logbuf.p;
   S.n_verts = s->nv;
   S.n_tets = s->nt;
-  S.n_tet_blocks = s->ntb;
+  S.layout = s->layout;
+  S.n_patches = s->n_patches;
+  S.n_partials = s->layout == XPBD_LAYOUT_PATCH ? s->n_patches : s->ntb;
   S.tet_block = s->bs_t;
   fill_scene_params(s, S, s->prm.dt / s->prm.substeps_num, s->prm.damping);
   s->table_valid = false;
@@ -414,15 +506,27 @@ int events_collect(xpbd_solver* s) {
 int enqueue_iterations(xpbd_solver* s, bool capturing) {
   const SceneDev& S = s->host_scene;
   const SceneDev* T = s->table.p;
+  const bool patch = s->layout == XPBD_LAYOUT_PATCH;
   for (int it = 0; it < S.iters; ++it) {
     std::pair<hipEvent_t, hipEvent_t> ev;
     const bool timed = s->timing && !capturing;
     if (timed) TRY(events_begin(s, ev));
-    HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
+    if (patch)
+      HIP_TRY(xpbd::launch_patch_project(T, s->pmap.p, s->n_patches, it == 0, s->stream));
+    else
+      HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
     if (timed) TRY(events_end(s, ev));
-    HIP_TRY(xpbd::launch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
-    if (S.cheb || S.monitor)
-      HIP_TRY(xpbd::launch_residual(T, s->tmap.p, s->ntb, s->bs_t, it, s->stream));
+    if (patch)
+      HIP_TRY(xpbd::launch_patch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
+    else
+      HIP_TRY(xpbd::launch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
+    if (S.cheb || S.monitor) {
+      if (patch)
+        HIP_TRY(xpbd::launch_patch_residual(T, s->pmap.p, s->n_patches, s->stream));
+      else
+        HIP_TRY(xpbd::launch_residual(T, s->tmap.p, s->ntb, s->bs_t, s->stream));
+      HIP_TRY(xpbd::launch_reduce(T, 1, it, s->stream));
+    }
     if (S.cheb) HIP_TRY(xpbd::launch_blend(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
   }
   return XPBD_OK;
@@ -444,7 +548,8 @@ int enqueue_substep(xpbd_solver* s, bool capturing) {
 }
 
 int graph_key_of(const SceneDev& S) {
-  return (S.iters << 4) | (S.cheb << 0) | (S.monitor << 1) | (S.block_mode << 2) | (S.csr << 3);
+  return (S.iters << 5) | (S.cheb << 0) | (S.monitor << 1) | (S.block_mode << 2) | (S.csr << 3) |
+         (S.layout << 4);
 }
 
 int fetch_log(xpbd_solver* s) {
@@ -536,6 +641,7 @@ int xpbd_desc_init(xpbd_desc* d) {
   d->reorder = 1;
   d->block_size = 256;
   d->device = 0;
+  d->layout = XPBD_LAYOUT_PATCH;
   return XPBD_OK;
 }
 
@@ -660,16 +766,44 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
   TRY(s->xj.upload(h_zero.data(), nv));
   TRY(s->dacc.upload(h_zero.data(), nv));
   TRY(s->valence.upload(val.data(), nv));
-  TRY(s->tet.upload(h_tet.data(), nt));
-  TRY(s->dm0.upload(h_dm0.data(), nt));
-  TRY(s->dm1.upload(h_dm1.data(), nt));
-  TRY(s->dm2.upload(h_dm2.data(), nt));
+  {
+    std::vector<float4> h_rest(nv);
+    for (int i = 0; i < nv; ++i)
+      h_rest[i] = make_float4(x[3 * size_t(i)], x[3 * size_t(i) + 1], x[3 * size_t(i) + 2], 0.f);
+    TRY(s->rest.upload(h_rest.data(), nv));
+  }
+  s->layout = d->layout;
+  if (d->layout == XPBD_LAYOUT_FLAT) {
+    TRY(s->tet.upload(h_tet.data(), nt));
+    TRY(s->dm0.upload(h_dm0.data(), nt));
+    TRY(s->dm1.upload(h_dm1.data(), nt));
+    TRY(s->dm2.upload(h_dm2.data(), nt));
+  } else {
+    Patches P;
+    build_patches(s->tets_new, nv, nt, P);
+    s->n_patches = int(P.tet_begin.size()) - 1;
+    TRY(s->p_tet_begin.upload(P.tet_begin.data(), P.tet_begin.size()));
+    TRY(s->p_vert_begin.upload(P.vert_begin.data(), P.vert_begin.size()));
+    TRY(s->p_vert_ids.upload(P.vert_ids.data(), P.vert_ids.size()));
+    TRY(s->tet_local.upload(P.tet_local.data(), P.tet_local.size()));
+    TRY(s->vslot_off.upload(P.vslot_off.data(), P.vslot_off.size()));
+    TRY(s->vslot.upload(P.vslot.data(), P.vslot.size()));
+    TRY(s->slot_acc.alloc(P.vert_ids.size()));
+    TRY(s->slot_acc.zero());
+    if (d->accumulate_mode == XPBD_ACCUM_CSR) {
+      TRY(s->lc_off.upload(P.lc_off.data(), P.lc_off.size()));
+      TRY(s->lc.upload(P.lc.data(), P.lc.size()));
+    }
+    std::vector<WorkItem> pm(s->n_patches);
+    for (int p = 0; p < s->n_patches; ++p) pm[p] = {0, p};
+    TRY(s->pmap.upload(pm.data(), pm.size()));
+  }
   TRY(s->lam.alloc(nt));
   TRY(s->lam.zero());
   TRY(s->d_old2new.upload(s->v_old2new.data(), nv));
   TRY(s->scratch_xyz.alloc(3 * size_t(nv)));
   TRY(s->flag.alloc(1));
-  if (d->accumulate_mode == XPBD_ACCUM_CSR) {
+  if (d->accumulate_mode == XPBD_ACCUM_CSR && d->layout == XPBD_LAYOUT_FLAT) {
     std::vector<int> adj(4 * size_t(nt));
     std::vector<int> fill(off_v.begin(), off_v.end() - 1);
     for (int t = 0; t < nt; ++t)
@@ -680,10 +814,8 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
     TRY(s->corner.zero());
   }
   TRY(build_maps(s.get()));
-  TRY(s->partial.alloc(3 * size_t(s->ntb)));
+  TRY(s->partial.alloc(3 * size_t(std::max(s->ntb, s->n_patches))));
   TRY(s->partial.zero());
-  TRY(s->ticket.alloc(1));
-  TRY(s->ticket.zero());
   SceneCtrl c0;
   std::memset(&c0, 0, sizeof(c0));
   c0.rho_hat = 0.999;  // neohookean_solver.py:65
@@ -1019,6 +1151,36 @@ int xpbd_get_kernel_time(xpbd_solver* s, double* total_ms, int64_t* launches) {
   TRY(events_collect(s));
   if (total_ms) *total_ms = s->timed_ms;
   if (launches) *launches = s->timed_launches;
+  return XPBD_OK;
+}
+
+int xpbd_patch_stats(const int32_t* tets, int32_t n_tets, int32_t n_verts, int32_t* n_patches,
+                     int64_t* n_slots, int32_t* max_patch_verts, int32_t* max_patch_tets) {
+  if (!tets || n_tets < 0 || n_verts <= 0) return set_error(XPBD_EINVAL, "bad mesh");
+  for (size_t i = 0; i < 4 * size_t(n_tets); ++i)
+    if (tets[i] < 0 || tets[i] >= n_verts) return set_error(XPBD_EINVAL, "vertex id out of range");
+  std::vector<int> t(tets, tets + 4 * size_t(n_tets));
+  Patches P;
+  build_patches(t, n_verts, n_tets, P);
+  const int np = int(P.tet_begin.size()) - 1;
+  int mv = 0, mt = 0;
+  for (int p = 0; p < np; ++p) {
+    mv = std::max(mv, P.vert_begin[p + 1] - P.vert_begin[p]);
+    mt = std::max(mt, P.tet_begin[p + 1] - P.tet_begin[p]);
+  }
+  // every tet's local ids must map back to its own vertices
+  for (int p = 0; p < np; ++p)
+    for (int tt = P.tet_begin[p]; tt < P.tet_begin[p + 1]; ++tt)
+      for (int k = 0; k < 4; ++k) {
+        const int l = int((P.tet_local[tt] >> (8 * k)) & 0xffu);
+        if (P.vert_begin[p] + l >= P.vert_begin[p + 1] ||
+            P.vert_ids[P.vert_begin[p] + l] != t[4 * size_t(tt) + k])
+          return set_error(XPBD_ESTATE, "patch %d tet %d corner %d maps to the wrong vertex", p, tt, k);
+      }
+  if (n_patches) *n_patches = np;
+  if (n_slots) *n_slots = int64_t(P.vert_ids.size());
+  if (max_patch_verts) *max_patch_verts = mv;
+  if (max_patch_tets) *max_patch_tets = mt;
   return XPBD_OK;
 }
 

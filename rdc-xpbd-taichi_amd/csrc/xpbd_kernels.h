@@ -15,7 +15,14 @@ hipError_t launch_project(const SceneDev* scenes, const WorkItem* tmap, int nblo
 hipError_t launch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                           hipStream_t st);
 hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nblocks, int bs,
-                           int it, hipStream_t st);
+                           hipStream_t st);
+hipError_t launch_patch_project(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
+                                int first, hipStream_t st);
+hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks,
+                                int bs, hipStream_t st);
+hipError_t launch_patch_residual(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
+                                 hipStream_t st);
+hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st);
 hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                         hipStream_t st);
 hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,

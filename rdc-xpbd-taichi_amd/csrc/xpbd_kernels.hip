@@ -1,10 +1,19 @@
-// gfx950 kernels of the XPBD Neo-Hookean substep (one launch per phase).
+// gfx950 kernels of the XPBD Neo-Hookean substep.
 //
-// Every kernel walks a work map: workgroup b handles elements
-// [map[b].first, map[b].first + blockDim.x) of scene map[b].scene, whose
-// buffers and parameters are read from a device-resident scene table.  One
-// solver is a table of one scene; the batched form packs many ragged scenes
-// into the same launches.
+// Every kernel walks a work map: workgroup b handles item map[b] of scene
+// map[b].scene (a range of vertices or tets, or one patch), whose buffers and
+// parameters come from a device-resident scene table.  One solver is a table
+// of one scene; the batched form packs many ragged scenes into the same
+// launches.
+//
+// Two tet layouts share the per-tet math:
+//  flat   one thread per tet; inv_Dm/V0 streamed from HBM; corrections go to
+//         per-vertex float atomics or to per-corner slots gathered per vertex.
+//  patch  one workgroup per patch (<= 256 Morton-contiguous tets, <= 256
+//         vertices); the patch's vertices are staged in LDS, inv_Dm/V0 are
+//         recomputed from the staged rest positions, corrections accumulate in
+//         LDS and leave the workgroup as one partial per (patch, vertex) slot,
+//         so no global atomics are issued.
 #include "xpbd_device.h"
 #include "xpbd_kernels.h"
 
@@ -12,14 +21,17 @@ namespace xpbd {
 
 namespace {
 
+constexpr int kPatchThreads = 256;  // threads per patch workgroup = max tets per patch
+constexpr int kPatchVerts = 256;    // max vertices per patch (u8 local ids)
+
 __device__ __forceinline__ void atomic_add3(float4* dst, float3 v) {
   unsafeAtomicAdd(&dst->x, v.x);
   unsafeAtomicAdd(&dst->y, v.y);
   unsafeAtomicAdd(&dst->z, v.z);
 }
 
-// Sum of a float over the 64 lanes of a wavefront (butterfly through DPP /
-// ds_swizzle lane exchanges); every lane receives the total.
+// Sum over the 64 lanes of a wavefront (lane-exchange butterfly); every lane
+// receives the total, in the same order on every run.
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -30,6 +42,167 @@ __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+struct TetProjection {
+  float3 dp[4];
+  float cnt[4];
+};
+
+// The per-tet constraint projection of one Jacobi iteration.
+//  block mode: solve_neohookean_block (neohookean_solver.py:358-451)
+//  split mode: solve_hydrostatic (:196-254) then solve_deviatoric (:144-194)
+//              on the same positions.
+__device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4], const Mat3& idm,
+                                            float V0, float2& lam, TetProjection& out) {
+  const float w[4] = {q[0].w, q[1].w, q[2].w, q[3].w};
+  const Mat3 F = deformation_gradient(f3(q[0]), f3(q[1]), f3(q[2]), f3(q[3]), idm);
+  const float dt = S.dt;
+  if (S.block_mode) {
+    const float C_D = norm_sqr(F) - 3.f;
+    const float C_H = det3(F) - 1.f;
+    Mat3 gfd;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * 2.f;
+    float3 gD[4], gH[4];
+    vertex_grads(gfd, idm, gD);
+    vertex_grads(ddet_dF(F), idm, gH);
+    float A11 = w[0] * dot(gD[0], gD[0]) + w[1] * dot(gD[1], gD[1]) + w[2] * dot(gD[2], gD[2]) +
+                w[3] * dot(gD[3], gD[3]);
+    float A22 = w[0] * dot(gH[0], gH[0]) + w[1] * dot(gH[1], gH[1]) + w[2] * dot(gH[2], gH[2]) +
+                w[3] * dot(gH[3], gH[3]);
+    const float A12 = w[0] * dot(gD[0], gH[0]) + w[1] * dot(gD[1], gH[1]) +
+                      w[2] * dot(gD[2], gH[2]) + w[3] * dot(gD[3], gH[3]);
+    const float aD = (1.f / (S.lame2 * V0)) / (dt * dt);
+    const float aH = (1.f / (S.lame1 * V0)) / (dt * dt);
+    A11 += aD;
+    A22 += aH;
+    const float b1 = -(C_D + aD * lam.x);
+    const float b2 = -(C_H + aH * lam.y);
+    float dlD, dlH;
+    solve2x2_with_fallback(A11, A12, A22, b1, b2, dlD, dlH);
+    lam.x += dlD;
+    lam.y += dlH;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      out.dp[k] = w[k] * (gD[k] * dlD + gH[k] * dlH);
+      out.cnt[k] = 1.f;
+    }
+  } else {
+    float3 gH[4], gD[4];
+    vertex_grads(ddet_dF(F), idm, gH);
+    const float C_H = det3(F) - 1.f;
+    float wsH = w[0] * dot(gH[0], gH[0]);
+    wsH += w[1] * dot(gH[1], gH[1]);
+    wsH += w[2] * dot(gH[2], gH[2]);
+    wsH += w[3] * dot(gH[3], gH[3]);
+    float dlH = 0.f;
+    const int okH = wsH > kEps;
+    if (okH) {
+      const float at = (1.f / (S.lame1 * V0)) / (dt * dt);
+      dlH = -(C_H + at * lam.y) / (wsH + at);
+      dlH *= S.hydro_relax;
+      lam.y += dlH;
+    }
+    Mat3 gfd;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) gfd.m[i][j] = 2.f * F.m[i][j];
+    vertex_grads(gfd, idm, gD);
+    const float C_D = norm_sqr(F) - 3.f;
+    float wsD = w[0] * dot(gD[0], gD[0]);
+    wsD += w[1] * dot(gD[1], gD[1]);
+    wsD += w[2] * dot(gD[2], gD[2]);
+    wsD += w[3] * dot(gD[3], gD[3]);
+    float dlD = 0.f;
+    const int okD = wsD > kEps;
+    if (okD) {
+      const float at = (1.f / (S.lame2 * V0)) / (dt * dt);
+      dlD = -(C_D + at * lam.x) / (wsD + at);
+      dlD *= S.dev_relax;
+      lam.x += dlD;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float3 a = make_float3(0.f, 0.f, 0.f);
+      if (okH) a = a + gH[k] * (dlH * w[k]);
+      if (okD) a = a + gD[k] * (dlD * w[k]);
+      out.dp[k] = a;
+      out.cnt[k] = float(okH + okD);
+    }
+  }
+}
+
+// compute_xpbd_residual (neohookean_solver.py:298-329) for one tet, plus the
+// |C_H| and |C_D| of the monitor kernels (:256-296).
+__device__ __forceinline__ void residual_tet(const SceneDev& S, float3 x0, float3 x1, float3 x2,
+                                             float3 x3, const Mat3& idm, float V0, float2 lam,
+                                             float& r, float& ah, float& ad) {
+  const Mat3 F = deformation_gradient(x0, x1, x2, x3, idm);
+  const float C_D = norm_sqr(F) - 3.f;
+  const float C_H = det3(F) - 1.f;
+  r = 0.f;
+  if (S.cheb) {
+    const float dt = S.dt;
+    const float aD = (1.f / (S.lame2 * V0)) / (dt * dt);
+    const float aH = (1.f / (S.lame1 * V0)) / (dt * dt);
+    const float rD = C_D + aD * lam.x;
+    const float rH = C_H + aH * lam.y;
+    r = sqrtf(rD * rD + rH * rH);
+  }
+  ah = fabsf(C_H);
+  ad = fabsf(C_D);
+}
+
+// Workgroup sum of (r, |C_H|, |C_D|) in a fixed order, written to partial[idx].
+__device__ __forceinline__ void store_partial(const SceneDev& S, int idx, float r, float ah,
+                                              float ad) {
+  __shared__ float red[3][16];
+  r = wave_sum(r);
+  ah = wave_sum(ah);
+  ad = wave_sum(ad);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  if (lane == 0) {
+    red[0][wave] = r;
+    red[1][wave] = ah;
+    red[2][wave] = ad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < nwave; ++i) {
+      s0 += red[0][i];
+      s1 += red[1][i];
+      s2 += red[2][i];
+    }
+    double* part = S.partial + 3 * size_t(idx);
+    part[0] = s0;
+    part[1] = s1;
+    part[2] = s2;
+  }
+}
+
+// x_J = y + delta_x / cons_num for one vertex (post_solve,
+// neohookean_solver.py:115-119), with delta_x summed from the vertex's patch
+// slots in ascending patch order.
+__device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int b = S.vslot_off[v], e = S.vslot_off[v + 1];
+  for (int a = b; a < e; ++a) {
+    const float4 c = S.slot_acc[S.vslot[a]];
+    acc.x += c.x;
+    acc.y += c.y;
+    acc.z += c.z;
+    acc.w += c.w;
+  }
+  const float cnt = S.block_mode ? S.valence[v] : acc.w;
+  const float4 p = S.pos[v];
+  float3 x = f3(p);
+  if (cnt > 0.f) x = x + make_float3(acc.x / cnt, acc.y / cnt, acc.z / cnt);
+  return make_float4(x.x, x.y, x.z, p.w);
 }
 
 }  // namespace
@@ -61,128 +234,41 @@ __global__ void k_predict(const SceneDev* __restrict__ scenes, const WorkItem* _
   if (save_prev) S.prev[v] = p;
 }
 
-// Per-tet projection of one Jacobi iteration.
-//  block mode: solve_neohookean_block (neohookean_solver.py:358-451)
-//  split mode: solve_hydrostatic (:196-254) then solve_deviatoric (:144-194),
-//              evaluated on the same positions and summed per vertex.
-// `first` marks the first iteration of a substep: the multipliers start at 0
-// (neohookean_solver.py:458-459) and the substep's Chebyshev control resets
-// (:461, :466).
+// ---- flat layout ---------------------------------------------------------------
+
+// One thread per tet.  `first` marks the first iteration of a substep: the
+// multipliers start at 0 (neohookean_solver.py:458-459).
 __global__ void k_project(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map,
                           int first) {
   const WorkItem wi = map[blockIdx.x];
   const SceneDev& S = scenes[wi.scene];
-  if (first && wi.first == 0 && threadIdx.x == 0) {
-    S.ctrl->have_prev = 0;
-    S.ctrl->omega = 1.0;
-  }
   const int t = wi.first + threadIdx.x;
   if (t >= S.n_tets) return;
   const int4 id = S.tet[t];
-  const float4 q0 = S.pos[id.x], q1 = S.pos[id.y], q2 = S.pos[id.z], q3 = S.pos[id.w];
+  const float4 q[4] = {S.pos[id.x], S.pos[id.y], S.pos[id.z], S.pos[id.w]};
   const float2 d2 = S.dm2[t];
   const Mat3 idm = load_inv_dm(S.dm0[t], S.dm1[t], d2);
-  const float V0 = d2.y;
   float2 lam = first ? make_float2(0.f, 0.f) : S.lam[t];
-  const float w[4] = {q0.w, q1.w, q2.w, q3.w};
-  const Mat3 F = deformation_gradient(f3(q0), f3(q1), f3(q2), f3(q3), idm);
-  const float dt = S.dt;
-  float3 dp[4];
-  float cnt[4] = {1.f, 1.f, 1.f, 1.f};
-  if (S.block_mode) {
-    const float C_D = norm_sqr(F) - 3.f;
-    const float C_H = det3(F) - 1.f;
-    Mat3 gfd;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * 2.f;
-    float3 gD[4], gH[4];
-    vertex_grads(gfd, idm, gD);
-    vertex_grads(ddet_dF(F), idm, gH);
-    float A11 = w[0] * dot(gD[0], gD[0]) + w[1] * dot(gD[1], gD[1]) + w[2] * dot(gD[2], gD[2]) +
-                w[3] * dot(gD[3], gD[3]);
-    float A22 = w[0] * dot(gH[0], gH[0]) + w[1] * dot(gH[1], gH[1]) + w[2] * dot(gH[2], gH[2]) +
-                w[3] * dot(gH[3], gH[3]);
-    const float A12 = w[0] * dot(gD[0], gH[0]) + w[1] * dot(gD[1], gH[1]) +
-                      w[2] * dot(gD[2], gH[2]) + w[3] * dot(gD[3], gH[3]);
-    const float aD = (1.f / (S.lame2 * V0)) / (dt * dt);
-    const float aH = (1.f / (S.lame1 * V0)) / (dt * dt);
-    A11 += aD;
-    A22 += aH;
-    const float b1 = -(C_D + aD * lam.x);
-    const float b2 = -(C_H + aH * lam.y);
-    float dlD, dlH;
-    solve2x2_with_fallback(A11, A12, A22, b1, b2, dlD, dlH);
-    lam.x += dlD;
-    lam.y += dlH;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) dp[k] = w[k] * (gD[k] * dlD + gH[k] * dlH);
-  } else {
-    // hydrostatic constraint C_H = det F - 1
-    float3 gH[4], gD[4];
-    vertex_grads(ddet_dF(F), idm, gH);
-    const float C_H = det3(F) - 1.f;
-    float wsH = w[0] * dot(gH[0], gH[0]);
-    wsH += w[1] * dot(gH[1], gH[1]);
-    wsH += w[2] * dot(gH[2], gH[2]);
-    wsH += w[3] * dot(gH[3], gH[3]);
-    float dlH = 0.f;
-    const int okH = wsH > kEps;
-    if (okH) {
-      const float at = (1.f / (S.lame1 * V0)) / (dt * dt);
-      dlH = -(C_H + at * lam.y) / (wsH + at);
-      dlH *= S.hydro_relax;
-      lam.y += dlH;
-    }
-    // deviatoric constraint C_D = |F|^2 - 3
-    Mat3 gfd;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) gfd.m[i][j] = 2.f * F.m[i][j];
-    vertex_grads(gfd, idm, gD);
-    const float C_D = norm_sqr(F) - 3.f;
-    float wsD = w[0] * dot(gD[0], gD[0]);
-    wsD += w[1] * dot(gD[1], gD[1]);
-    wsD += w[2] * dot(gD[2], gD[2]);
-    wsD += w[3] * dot(gD[3], gD[3]);
-    float dlD = 0.f;
-    const int okD = wsD > kEps;
-    if (okD) {
-      const float at = (1.f / (S.lame2 * V0)) / (dt * dt);
-      dlD = -(C_D + at * lam.x) / (wsD + at);
-      dlD *= S.dev_relax;
-      lam.x += dlD;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      float3 a = make_float3(0.f, 0.f, 0.f);
-      if (okH) a = a + gH[k] * (dlH * w[k]);
-      if (okD) a = a + gD[k] * (dlD * w[k]);
-      dp[k] = a;
-      cnt[k] = float(okH + okD);
-    }
-  }
+  TetProjection P;
+  project_tet(S, q, idm, d2.y, lam, P);
   S.lam[t] = lam;
   if (S.csr) {
     float4* c = S.corner + 4 * size_t(t);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) c[k] = make_float4(dp[k].x, dp[k].y, dp[k].z, cnt[k]);
+    for (int k = 0; k < 4; ++k) c[k] = make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
   } else {
     const int ids[4] = {id.x, id.y, id.z, id.w};
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      atomic_add3(S.dacc + ids[k], dp[k]);
-      if (!S.block_mode && cnt[k] != 0.f) unsafeAtomicAdd(&S.dacc[ids[k]].w, cnt[k]);
+      atomic_add3(S.dacc + ids[k], P.dp[k]);
+      if (!S.block_mode && P.cnt[k] != 0.f) unsafeAtomicAdd(&S.dacc[ids[k]].w, P.cnt[k]);
     }
   }
 }
 
 // pre_solve / post_solve (neohookean_solver.py:109-119): x_J = y + delta_x /
 // cons_num.  With Chebyshev on, x_J goes to xj (y stays in pos for the blend);
-// otherwise pos is updated in place.  Atomic accumulators are cleared here
-// for the next iteration.
+// otherwise pos is updated in place.  Atomic accumulators are cleared here.
 __global__ void k_average(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map) {
   const WorkItem wi = map[blockIdx.x];
   const SceneDev& S = scenes[wi.scene];
@@ -201,7 +287,7 @@ __global__ void k_average(const SceneDev* __restrict__ scenes, const WorkItem* _
       acc.w += c.w;
     }
     d = f3(acc);
-    cnt = acc.w;
+    cnt = S.block_mode ? S.valence[v] : acc.w;
   } else {
     const float4 acc = S.dacc[v];
     d = f3(acc);
@@ -218,12 +304,8 @@ __global__ void k_average(const SceneDev* __restrict__ scenes, const WorkItem* _
     S.pos[v] = out;
 }
 
-// compute_xpbd_residual (neohookean_solver.py:298-329) and the monitor sums
-// of compute_hydrostatic_error / compute_deviatoric_error (:256-296) on x_J;
-// the last workgroup of each scene then performs the host-side rho/omega
-// update of solve_step (:492-504, :537-546) in double on the device.
-__global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map,
-                           int it) {
+// Residual per tet on x_J; one partial sum per workgroup.
+__global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map) {
   const WorkItem wi = map[blockIdx.x];
   const SceneDev& S = scenes[wi.scene];
   const int t = wi.first + threadIdx.x;
@@ -233,56 +315,140 @@ __global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* 
     const int4 id = S.tet[t];
     const float2 d2 = S.dm2[t];
     const Mat3 idm = load_inv_dm(S.dm0[t], S.dm1[t], d2);
-    const Mat3 F = deformation_gradient(f3(X[id.x]), f3(X[id.y]), f3(X[id.z]), f3(X[id.w]), idm);
-    const float C_D = norm_sqr(F) - 3.f;
-    const float C_H = det3(F) - 1.f;
-    if (S.cheb) {
-      const float2 lam = S.lam[t];
-      const float dt = S.dt;
-      const float aD = (1.f / (S.lame2 * d2.y)) / (dt * dt);
-      const float aH = (1.f / (S.lame1 * d2.y)) / (dt * dt);
-      const float rD = C_D + aD * lam.x;
-      const float rH = C_H + aH * lam.y;
-      r = sqrtf(rD * rD + rH * rH);
-    }
-    ah = fabsf(C_H);
-    ad = fabsf(C_D);
+    residual_tet(S, f3(X[id.x]), f3(X[id.y]), f3(X[id.z]), f3(X[id.w]), idm, d2.y, S.lam[t], r,
+                 ah, ad);
   }
-  // workgroup sum: wavefront butterfly, then one value per wave through LDS
-  __shared__ float red[3][16];
-  __shared__ int last;
-  r = wave_sum(r);
-  ah = wave_sum(ah);
-  ad = wave_sum(ad);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  if (lane == 0) {
-    red[0][wave] = r;
-    red[1][wave] = ah;
-    red[2][wave] = ad;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = 0; i < nwave; ++i) {
-      s0 += red[0][i];
-      s1 += red[1][i];
-      s2 += red[2][i];
-    }
-    double* part = S.partial + 3 * size_t(wi.first / S.tet_block);
-    part[0] = s0;
-    part[1] = s1;
-    part[2] = s2;
-    const unsigned done =
-        __hip_atomic_fetch_add(S.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    last = (done == unsigned(S.n_tet_blocks - 1));
+  store_partial(S, wi.first / S.tet_block, r, ah, ad);
+}
+
+// ---- patch layout ----------------------------------------------------------------
+
+// One workgroup per patch: stage the patch's vertices in LDS, project every
+// tet, accumulate the corrections per patch vertex in LDS, then write one
+// partial per (patch, vertex) slot.
+__global__ void __launch_bounds__(kPatchThreads)
+    k_patch_project(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map,
+                    int first) {
+  const WorkItem wi = map[blockIdx.x];
+  const SceneDev& S = scenes[wi.scene];
+  const int p = wi.first;
+  const int tb = S.p_tet_begin[p], nt = S.p_tet_begin[p + 1] - tb;
+  const int vb = S.p_vert_begin[p], nv = S.p_vert_begin[p + 1] - vb;
+  __shared__ float4 s_y[kPatchVerts];
+  __shared__ float4 s_rest[kPatchVerts];
+  __shared__ float4 s_acc[kPatchVerts];
+  __shared__ float4 s_corner[4 * kPatchThreads];
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const int g = S.p_vert_ids[vb + i];
+    s_y[i] = S.pos[g];
+    s_rest[i] = S.rest[g];
+    s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // Deterministic fixed-order sum of this scene's partials: strided per
-  // thread, butterfly per wave, then waves in order.
+  const int lt = threadIdx.x;
+  if (lt < nt) {
+    const int t = tb + lt;
+    const unsigned loc = S.tet_local[t];
+    const int l[4] = {int(loc & 0xffu), int((loc >> 8) & 0xffu), int((loc >> 16) & 0xffu),
+                      int(loc >> 24)};
+    const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
+    Mat3 idm;
+    float V0;
+    rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
+    float2 lam = first ? make_float2(0.f, 0.f) : S.lam[t];
+    TetProjection P;
+    project_tet(S, q, idm, V0, lam, P);
+    S.lam[t] = lam;
+    if (S.csr) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        s_corner[4 * lt + k] = make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        atomicAdd(&s_acc[l[k]].x, P.dp[k].x);
+        atomicAdd(&s_acc[l[k]].y, P.dp[k].y);
+        atomicAdd(&s_acc[l[k]].z, P.dp[k].z);
+        if (!S.block_mode) atomicAdd(&s_acc[l[k]].w, P.cnt[k]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    float4 a;
+    if (S.csr) {
+      a = make_float4(0.f, 0.f, 0.f, 0.f);
+      const int cb = S.lc_off[vb + i], ce = S.lc_off[vb + i + 1];
+      for (int c = cb; c < ce; ++c) {
+        const float4 v = s_corner[S.lc[c]];
+        a.x += v.x;
+        a.y += v.y;
+        a.z += v.z;
+        a.w += v.w;
+      }
+    } else {
+      a = s_acc[i];
+    }
+    S.slot_acc[vb + i] = a;
+  }
+}
+
+// Jacobi average from the slots (cheb: into xj; otherwise pos in place).
+__global__ void k_patch_average(const SceneDev* __restrict__ scenes,
+                                const WorkItem* __restrict__ map) {
+  const WorkItem wi = map[blockIdx.x];
+  const SceneDev& S = scenes[wi.scene];
+  const int v = wi.first + threadIdx.x;
+  if (v >= S.n_verts) return;
+  const float4 out = jacobi_average_slots(S, v);
+  if (S.cheb)
+    S.xj[v] = out;
+  else
+    S.pos[v] = out;
+}
+
+// Residual per patch: stage x_J and rest positions of the patch in LDS.
+__global__ void __launch_bounds__(kPatchThreads)
+    k_patch_residual(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map) {
+  const WorkItem wi = map[blockIdx.x];
+  const SceneDev& S = scenes[wi.scene];
+  const int p = wi.first;
+  const int tb = S.p_tet_begin[p], nt = S.p_tet_begin[p + 1] - tb;
+  const int vb = S.p_vert_begin[p], nv = S.p_vert_begin[p + 1] - vb;
+  const float4* X = S.cheb ? S.xj : S.pos;
+  __shared__ float4 s_x[kPatchVerts];
+  __shared__ float4 s_rest[kPatchVerts];
+  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+    const int g = S.p_vert_ids[vb + i];
+    s_x[i] = X[g];
+    s_rest[i] = S.rest[g];
+  }
+  __syncthreads();
+  float r = 0.f, ah = 0.f, ad = 0.f;
+  const int lt = threadIdx.x;
+  if (lt < nt) {
+    const int t = tb + lt;
+    const unsigned loc = S.tet_local[t];
+    const int l[4] = {int(loc & 0xffu), int((loc >> 8) & 0xffu), int((loc >> 16) & 0xffu),
+                      int(loc >> 24)};
+    Mat3 idm;
+    float V0;
+    rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
+    residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, V0, S.lam[t],
+                 r, ah, ad);
+  }
+  store_partial(S, p, r, ah, ad);
+}
+
+// ---- per-scene scalar control ----------------------------------------------------
+
+// One workgroup per scene: fixed-order sum of the scene's partials, then the
+// residual-driven rho estimate and Chebyshev omega of solve_step
+// (neohookean_solver.py:492-504, 537-546) in double.
+__global__ void k_reduce(const SceneDev* __restrict__ scenes, int it) {
+  const SceneDev& S = scenes[blockIdx.x];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int b = threadIdx.x; b < S.n_tet_blocks; b += blockDim.x) {
+  for (int b = threadIdx.x; b < S.n_partials; b += blockDim.x) {
     const double* part = S.partial + 3 * size_t(b);
     s0 += part[0];
     s1 += part[1];
@@ -292,13 +458,14 @@ __global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* 
   s1 = wave_sum_d(s1);
   s2 = wave_sum_d(s2);
   __shared__ double dred[3][16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   if (lane == 0) {
     dred[0][wave] = s0;
     dred[1][wave] = s1;
     dred[2][wave] = s2;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || it >= S.iters) return;
   double tot[3] = {0.0, 0.0, 0.0};
   for (int i = 0; i < nwave; ++i) {
     tot[0] += dred[0][i];
@@ -306,6 +473,10 @@ __global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* 
     tot[2] += dred[2][i];
   }
   SceneCtrl* c = S.ctrl;
+  if (it == 0) {  // prev_r_hat = None and omega = 1.0 per substep (:461, :466)
+    c->have_prev = 0;
+    c->omega = 1.0;
+  }
   const double cnt = double(S.n_tets);
   double r_hat = 0.0;
   double omega = 1.0;
@@ -337,7 +508,6 @@ __global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* 
   L[3] = omega;
   L[4] = S.monitor && S.n_tets > 0 ? tot[1] / cnt : 0.0;
   L[5] = S.monitor && S.n_tets > 0 ? tot[2] / cnt : 0.0;
-  __hip_atomic_store(S.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // _chebyshev_update + _rotate_prev_with_save (neohookean_solver.py:131-142).
@@ -376,7 +546,7 @@ __global__ void k_advance(const SceneDev* __restrict__ scenes, const WorkItem* _
 }
 
 // Copies between a caller-order (V,3) float array and an internal float4
-// stream: dst[perm[i]] <- src[i] (scatter) or dst[i] <- src[perm[i]] (gather).
+// stream: dst[old_to_new[i]] <- src[i] (scatter) or dst[i] <- src[old_to_new[i]].
 __global__ void k_scatter_xyz(float4* __restrict__ dst, const float* __restrict__ src,
                               const int* __restrict__ old_to_new, int n, int keep_w) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -396,7 +566,7 @@ __global__ void k_gather_xyz(float* __restrict__ dst, const float4* __restrict__
   dst[3 * i + 2] = a.z;
 }
 
-// Returns in *flag a nonzero value when any position is not finite.
+// Sets *flag when any position is not finite.
 __global__ void k_check_finite(const float4* __restrict__ pos, int n, int* flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -422,8 +592,27 @@ hipError_t launch_average(const SceneDev* scenes, const WorkItem* vmap, int nblo
   return hipGetLastError();
 }
 hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nblocks, int bs,
-                           int it, hipStream_t st) {
-  k_residual<<<nblocks, bs, 0, st>>>(scenes, tmap, it);
+                           hipStream_t st) {
+  k_residual<<<nblocks, bs, 0, st>>>(scenes, tmap);
+  return hipGetLastError();
+}
+hipError_t launch_patch_project(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
+                                int first, hipStream_t st) {
+  k_patch_project<<<nblocks, kPatchThreads, 0, st>>>(scenes, pmap, first);
+  return hipGetLastError();
+}
+hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks,
+                                int bs, hipStream_t st) {
+  k_patch_average<<<nblocks, bs, 0, st>>>(scenes, vmap);
+  return hipGetLastError();
+}
+hipError_t launch_patch_residual(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
+                                 hipStream_t st) {
+  k_patch_residual<<<nblocks, kPatchThreads, 0, st>>>(scenes, pmap);
+  return hipGetLastError();
+}
+hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st) {
+  k_reduce<<<n_scenes, 256, 0, st>>>(scenes, it);
   return hipGetLastError();
 }
 hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,

@@ -20,7 +20,7 @@ XPBD_ENOMEM = -3
 XPBD_ESTATE = -4
 XPBD_ENUMERIC = -5
 XPBD_MAX_SDF = 8
-XPBD_ABI_VERSION = 1
+XPBD_ABI_VERSION = 2
 
 ACCUM_ATOMIC = 0
 ACCUM_CSR = 1
@@ -28,6 +28,8 @@ SDF_PLANE = 0
 SDF_SPHERE = 1
 PATH_KERNELS = 0
 PATH_GRAPH = 1
+LAYOUT_FLAT = 0
+LAYOUT_PATCH = 1
 
 # Every symbol the header declares; tests check that the library exports them.
 EXPORTED = (
@@ -41,6 +43,7 @@ EXPORTED = (
     "xpbd_get_stream", "xpbd_set_logging", "xpbd_log_size", "xpbd_get_log", "xpbd_clear_log",
     "xpbd_get_state", "xpbd_set_state", "xpbd_get_sizes", "xpbd_set_exec_path",
     "xpbd_set_kernel_timing", "xpbd_get_kernel_time", "xpbd_check_finite", "xpbd_last_error",
+    "xpbd_patch_stats",
 )
 
 
@@ -83,6 +86,7 @@ class XpbdDesc(ctypes.Structure):
         ("reorder", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
         ("device", ctypes.c_int32),
+        ("layout", ctypes.c_int32),
     ]
 
 
@@ -104,6 +108,15 @@ def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    if os.environ.get("XPBD_NO_TORCH") != "1":
+        # PyTorch-ROCm ships its own HIP runtime (SONAME libamdhip64.so.7).  When
+        # it is loaded first, libxpbd_hip.so binds to that same runtime, so torch
+        # tensors and this library share one device context; loaded the other way
+        # round the process would hold two HIP runtimes and torch would see no GPU.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python rdc-xpbd-taichi_amd/build.py` "
@@ -155,6 +168,9 @@ def load() -> ctypes.CDLL:
         "xpbd_get_kernel_time": [vp, dp, ctypes.POINTER(ctypes.c_int64)],
         "xpbd_check_finite": [vp],
         "xpbd_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+        "xpbd_patch_stats": [ctypes.POINTER(ctypes.c_int32), i32, i32,
+                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
     }
     for name, args in sigs.items():
         fn = getattr(lib, name)
@@ -193,3 +209,18 @@ def fptr(a: np.ndarray):
 
 def dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def patch_stats(tets: np.ndarray, n_verts: int) -> dict:
+    """Patch layout the library would build for these tets (host only)."""
+    lib = load()
+    t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    npch = ctypes.c_int32()
+    nsl = ctypes.c_int64()
+    mv = ctypes.c_int32()
+    mt = ctypes.c_int32()
+    check(lib.xpbd_patch_stats(t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), t.shape[0],
+                               int(n_verts), ctypes.byref(npch), ctypes.byref(nsl),
+                               ctypes.byref(mv), ctypes.byref(mt)))
+    return {"n_patches": npch.value, "n_slots": nsl.value, "max_patch_verts": mv.value,
+            "max_patch_tets": mt.value}

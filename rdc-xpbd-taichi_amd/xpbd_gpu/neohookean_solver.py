@@ -14,6 +14,7 @@ Execution options with no reference counterpart are keyword-only:
   reorder          Morton-reorder tets and vertices internally (default True)
   device           HIP device ordinal
   exec_path        "graph" (hipGraph per substep) or "kernels"
+  layout           "patch" (workgroup per LDS-staged patch, default) or "flat"
   log_residuals    keep r_hat / rho_hat / omega per iteration in
                    `residual_history` (one device sync per substep)
 """
@@ -39,6 +40,7 @@ def calc_second_lame(E, nu):
 
 _ACCUM = {"atomic": _abi.ACCUM_ATOMIC, "csr": _abi.ACCUM_CSR}
 _PATHS = {"graph": _abi.PATH_GRAPH, "kernels": _abi.PATH_KERNELS}
+_LAYOUTS = {"patch": _abi.LAYOUT_PATCH, "flat": _abi.LAYOUT_FLAT}
 
 # Python attributes that feed the device parameter block.
 _PARAM_ATTRS = ("young_modulus", "poisson_ratio", "frame_dt", "dt", "substeps_num",
@@ -74,6 +76,7 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
                  reorder=True,
                  device=0,
                  exec_path="graph",
+                 layout="patch",
                  log_residuals=False):
         self.constraints_iter = constraints_iter
         self.hydrostatic_relaxation = hydrostatic_relaxation
@@ -105,6 +108,9 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
             raise ValueError(f"accumulate_mode must be one of {sorted(_ACCUM)}")
         if exec_path not in _PATHS:
             raise ValueError(f"exec_path must be one of {sorted(_PATHS)}")
+        if layout not in _LAYOUTS:
+            raise ValueError(f"layout must be one of {sorted(_LAYOUTS)}")
+        self._layout = layout
         self._accumulate_mode = accumulate_mode
         self._reorder = bool(reorder)
         self._device = int(device)
@@ -164,6 +170,7 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         d.reorder = int(self._reorder)
         d.block_size = int(self.block_size)
         d.device = self._device
+        d.layout = _LAYOUTS[self._layout]
         return d
 
     def initialize_specific(self, scale, offset):

