@@ -67,7 +67,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=64, help="hexes per axis")
-    ap.add_argument("--accumulate", default="atomic", choices=["atomic", "csr"])
+    ap.add_argument("--accumulate", default="csr", choices=["atomic", "csr"])
+    ap.add_argument("--layout", default="patch", choices=["patch", "flat"])
     ap.add_argument("--exec-path", default="graph", choices=["graph", "kernels"])
     ap.add_argument("--cpu-substeps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -89,7 +90,8 @@ def main():
     print(f"[bench] workload cfg2: {V} verts, {T} tets, {subs} substeps x {iters} iterations "
           f"per frame; accumulate={args.accumulate} exec={args.exec_path}", file=sys.stderr,
           flush=True)
-    solver = make_solver(scene, accumulate_mode=args.accumulate, exec_path=args.exec_path)
+    solver = make_solver(scene, accumulate_mode=args.accumulate, exec_path=args.exec_path,
+                         layout=args.layout)
 
     for _ in range(args.warmup):
         solver.solve()
@@ -137,6 +139,7 @@ def main():
             "workload": f"cfg2 single large mesh: {args.n}^3 jittered box, {T} tets, {V} verts, "
                         f"{subs} substeps x {iters} iterations per step (one frame)",
             "accumulate_mode": args.accumulate,
+            "layout": args.layout,
             "exec_path": args.exec_path,
         },
         "hbm_gbps_algorithmic": gbps,

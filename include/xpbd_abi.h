@@ -85,7 +85,7 @@ typedef struct xpbd_desc {
   int32_t n_sdf;               /* >= 1 (xpbd_base.py:66-67) */
   const xpbd_sdf* sdfs;
   /* execution options (no reference counterpart) */
-  int32_t accumulate_mode;     /* XPBD_ACCUM_ATOMIC or XPBD_ACCUM_CSR */
+  int32_t accumulate_mode;     /* XPBD_ACCUM_CSR (default) or XPBD_ACCUM_ATOMIC */
   int32_t reorder;             /* 1: Morton-reorder tets and vertices internally */
   int32_t block_size;          /* threads per workgroup for flat tet kernels (64..1024) */
   int32_t device;              /* HIP device ordinal */

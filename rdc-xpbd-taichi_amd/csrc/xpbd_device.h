@@ -66,14 +66,12 @@ struct SceneDev {
   const float2* dm2;
   // patch layout: one workgroup per patch of <= 256 tets / <= 256 vertices
   const int* p_tet_begin;   // n_patches + 1
-  const int* p_vert_begin;  // n_patches + 1; also the slot ids of the patch vertices
-  const int* p_vert_ids;    // global vertex id of each slot
-  const unsigned* tet_local;  // 4 x u8 local vertex ids per tet
-  const int* vslot_off;     // n_verts + 1: slots of each vertex, ascending patch
-  const int* vslot;
-  float4* slot_acc;         // per-slot partial delta_x (xyz) and constraint count (w)
-  const int* lc_off;        // per slot: its corners in the patch, ascending tet
-  const unsigned short* lc; // corner = local tet * 4 + k
+  const int* p_vert_begin;  // n_patches + 1: range of the patch's vertex records
+  const int* p_vert_ids;    // per patch vertex: global vertex id
+  const int2* p_vert_sum;   // per patch vertex: (corner base | corner count << 16, slot)
+  const uint2* tet_local;   // per tet: 4 x u8 local vertex ids, 4 x u8 corner ranks
+  const int* vslot_off;     // n_verts + 1: each vertex's slots, contiguous, ascending patch
+  float4* slot_acc;         // per slot: partial delta_x (xyz) and constraint count (w)
   float2* lam;
   double* partial;  // 3 doubles per tet workgroup / patch: sum r, sum |C_H|, sum |C_D|
   SceneCtrl* ctrl;

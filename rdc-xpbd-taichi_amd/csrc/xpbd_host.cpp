@@ -176,70 +176,73 @@ constexpr int kPatchTets = 256;
 constexpr int kPatchVerts = 256;
 
 struct Patches {
-  std::vector<int> tet_begin, vert_begin, vert_ids, vslot_off, vslot, lc_off;
-  std::vector<unsigned> tet_local;
-  std::vector<unsigned short> lc;
+  std::vector<int> tet_begin, vert_begin, vert_ids, vslot_off;
+  std::vector<int2> vert_sum;     // (corner base | corner count << 16, slot)
+  std::vector<uint2> tet_local;   // (4 x u8 local ids, 4 x u8 corner ranks)
 };
 
 void build_patches(const std::vector<int>& tets, int nv, int nt, Patches& P) {
   P.tet_begin.assign(1, 0);
   P.vert_begin.assign(1, 0);
   P.vert_ids.clear();
-  P.tet_local.assign(nt, 0u);
-  P.lc_off.assign(1, 0);
-  P.lc.clear();
-  std::vector<int> stamp(nv, -1), local(nv, 0);
+  P.vert_sum.clear();
+  P.tet_local.assign(nt, make_uint2(0u, 0u));
+  std::vector<int> stamp(nv, -1), local(nv, 0), lcount(nv, 0);
   std::vector<int> cur;
+  std::vector<int> patch_of_slot;
   int start = 0;
   auto close = [&](int end) {
-    const int p = int(P.tet_begin.size()) - 1;
     std::sort(cur.begin(), cur.end());
     for (size_t i = 0; i < cur.size(); ++i) local[cur[i]] = int(i);
-    std::vector<std::vector<unsigned short>> corners(cur.size());
+    std::vector<int> cnt(cur.size(), 0);
     for (int t = start; t < end; ++t) {
-      unsigned packed = 0;
+      unsigned ids = 0, ranks = 0;
       for (int k = 0; k < 4; ++k) {
         const int l = local[tets[4 * size_t(t) + k]];
-        packed |= unsigned(l) << (8 * k);
-        corners[l].push_back((unsigned short)(4 * (t - start) + k));
+        ids |= unsigned(l) << (8 * k);
+        ranks |= unsigned(cnt[l]++) << (8 * k);  // corners of a vertex in tet order
       }
-      P.tet_local[t] = packed;
+      P.tet_local[t] = make_uint2(ids, ranks);
     }
+    int base = 0;
     for (size_t i = 0; i < cur.size(); ++i) {
       P.vert_ids.push_back(cur[i]);
-      P.lc.insert(P.lc.end(), corners[i].begin(), corners[i].end());
-      P.lc_off.push_back(int(P.lc.size()));
+      P.vert_sum.push_back(make_int2(base | (cnt[i] << 16), 0));
+      patch_of_slot.push_back(int(P.tet_begin.size()) - 1);
+      base += cnt[i];
     }
     P.tet_begin.push_back(end);
     P.vert_begin.push_back(int(P.vert_ids.size()));
-    (void)p;
     cur.clear();
     start = end;
   };
   for (int t = 0; t < nt; ++t) {
     const int p = int(P.tet_begin.size()) - 1;
-    int fresh = 0;
-    for (int k = 0; k < 4; ++k) fresh += stamp[tets[4 * size_t(t) + k]] != p;
-    if (t - start == kPatchTets || int(cur.size()) + fresh > kPatchVerts) {
-      close(t);
+    int fresh = 0, crowded = 0;
+    for (int k = 0; k < 4; ++k) {
+      const int g = tets[4 * size_t(t) + k];
+      fresh += stamp[g] != p;
+      crowded |= stamp[g] == p && lcount[g] >= 255;  // a rank must fit in a byte
     }
+    if (t - start == kPatchTets || int(cur.size()) + fresh > kPatchVerts || crowded) close(t);
     const int q = int(P.tet_begin.size()) - 1;
     for (int k = 0; k < 4; ++k) {
       const int g = tets[4 * size_t(t) + k];
       if (stamp[g] != q) {
         stamp[g] = q;
+        lcount[g] = 0;
         cur.push_back(g);
       }
+      lcount[g]++;
     }
   }
   if (nt > start) close(nt);
-  // vertex -> slots, ascending slot (= ascending patch)
+  // slots of a vertex are contiguous and in ascending patch order
   P.vslot_off.assign(nv + 1, 0);
   for (int g : P.vert_ids) P.vslot_off[g + 1]++;
   for (int v = 0; v < nv; ++v) P.vslot_off[v + 1] += P.vslot_off[v];
-  P.vslot.assign(P.vert_ids.size(), 0);
   std::vector<int> fill(P.vslot_off.begin(), P.vslot_off.end() - 1);
-  for (size_t sl = 0; sl < P.vert_ids.size(); ++sl) P.vslot[fill[P.vert_ids[sl]]++] = int(sl);
+  for (size_t i = 0; i < P.vert_ids.size(); ++i) P.vert_sum[i].y = fill[P.vert_ids[i]]++;
 }
 
 }  // namespace
@@ -257,9 +260,9 @@ struct xpbd_solver {
   std::vector<float> m_host, inv_m_host;  // internal order
 
   DevBuf<float4> pos, xs, vel, prev, xj, dacc, corner, rest, slot_acc;
-  DevBuf<int> p_tet_begin, p_vert_begin, p_vert_ids, vslot_off, vslot, lc_off;
-  DevBuf<unsigned> tet_local;
-  DevBuf<unsigned short> lc;
+  DevBuf<int> p_tet_begin, p_vert_begin, p_vert_ids, vslot_off;
+  DevBuf<int2> p_vert_sum;
+  DevBuf<uint2> tet_local;
   DevBuf<WorkItem> pmap;
   int n_patches = 0;
   int layout = XPBD_LAYOUT_PATCH;
@@ -370,12 +373,10 @@ int build_scene(xpbd_solver* s) {
   S.p_tet_begin = s->p_tet_begin.p;
   S.p_vert_begin = s->p_vert_begin.p;
   S.p_vert_ids = s->p_vert_ids.p;
+  S.p_vert_sum = s->p_vert_sum.p;
   S.tet_local = s->tet_local.p;
   S.vslot_off = s->vslot_off.p;
-  S.vslot = s->vslot.p;
   S.slot_acc = s->slot_acc.p;
-  S.lc_off = s->lc_off.p;
-  S.lc = s->lc.p;
   S.lam = s->lam.p;
   S.partial = s->partial.p;
   S.ctrl = s->ctrl.p;
@@ -516,10 +517,11 @@ int enqueue_iterations(xpbd_solver* s, bool capturing) {
     else
       HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
     if (timed) TRY(events_end(s, ev));
-    if (patch)
+    if (patch) {
       HIP_TRY(xpbd::launch_patch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
-    else
+    } else {
       HIP_TRY(xpbd::launch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
+    }
     if (S.cheb || S.monitor) {
       if (patch)
         HIP_TRY(xpbd::launch_patch_residual(T, s->pmap.p, s->n_patches, s->stream));
@@ -637,7 +639,7 @@ int xpbd_desc_init(xpbd_desc* d) {
   d->damping = 5.0;
   d->gravity[1] = -9.8;
   d->total_mass = 1.0;
-  d->accumulate_mode = XPBD_ACCUM_ATOMIC;
+  d->accumulate_mode = XPBD_ACCUM_CSR;
   d->reorder = 1;
   d->block_size = 256;
   d->device = 0;
@@ -785,15 +787,11 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
     TRY(s->p_tet_begin.upload(P.tet_begin.data(), P.tet_begin.size()));
     TRY(s->p_vert_begin.upload(P.vert_begin.data(), P.vert_begin.size()));
     TRY(s->p_vert_ids.upload(P.vert_ids.data(), P.vert_ids.size()));
+    TRY(s->p_vert_sum.upload(P.vert_sum.data(), P.vert_sum.size()));
     TRY(s->tet_local.upload(P.tet_local.data(), P.tet_local.size()));
     TRY(s->vslot_off.upload(P.vslot_off.data(), P.vslot_off.size()));
-    TRY(s->vslot.upload(P.vslot.data(), P.vslot.size()));
     TRY(s->slot_acc.alloc(P.vert_ids.size()));
     TRY(s->slot_acc.zero());
-    if (d->accumulate_mode == XPBD_ACCUM_CSR) {
-      TRY(s->lc_off.upload(P.lc_off.data(), P.lc_off.size()));
-      TRY(s->lc.upload(P.lc.data(), P.lc.size()));
-    }
     std::vector<WorkItem> pm(s->n_patches);
     for (int p = 0; p < s->n_patches; ++p) pm[p] = {0, p};
     TRY(s->pmap.upload(pm.data(), pm.size()));
@@ -1172,11 +1170,34 @@ int xpbd_patch_stats(const int32_t* tets, int32_t n_tets, int32_t n_verts, int32
   for (int p = 0; p < np; ++p)
     for (int tt = P.tet_begin[p]; tt < P.tet_begin[p + 1]; ++tt)
       for (int k = 0; k < 4; ++k) {
-        const int l = int((P.tet_local[tt] >> (8 * k)) & 0xffu);
+        const int l = int((P.tet_local[tt].x >> (8 * k)) & 0xffu);
         if (P.vert_begin[p] + l >= P.vert_begin[p + 1] ||
             P.vert_ids[P.vert_begin[p] + l] != t[4 * size_t(tt) + k])
           return set_error(XPBD_ESTATE, "patch %d tet %d corner %d maps to the wrong vertex", p, tt, k);
       }
+  // corner positions base(l) + rank cover [0, 4 * tets) once per patch, and
+  // each vertex's slot lies in its own contiguous range, once
+  std::vector<char> seen_slot(P.vert_ids.size(), 0);
+  for (int p = 0; p < np; ++p) {
+    const int tb = P.tet_begin[p], ntp = P.tet_begin[p + 1] - tb, vb = P.vert_begin[p];
+    std::vector<char> seen(4 * size_t(ntp), 0);
+    for (int tt = tb; tt < tb + ntp; ++tt)
+      for (int k = 0; k < 4; ++k) {
+        const int l = int((P.tet_local[tt].x >> (8 * k)) & 0xffu);
+        const int rank = int((P.tet_local[tt].y >> (8 * k)) & 0xffu);
+        const int base = P.vert_sum[vb + l].x & 0xffff, cnt = P.vert_sum[vb + l].x >> 16;
+        const int pos = base + rank;
+        if (rank >= cnt || pos >= 4 * ntp || seen[pos])
+          return set_error(XPBD_ESTATE, "patch %d: corner position %d invalid or reused", p, pos);
+        seen[pos] = 1;
+      }
+    for (int i = vb; i < P.vert_begin[p + 1]; ++i) {
+      const int g = P.vert_ids[i], dst = P.vert_sum[i].y;
+      if (dst < P.vslot_off[g] || dst >= P.vslot_off[g + 1] || seen_slot[dst])
+        return set_error(XPBD_ESTATE, "slot %d of vertex %d out of its range or reused", dst, g);
+      seen_slot[dst] = 1;
+    }
+  }
   if (n_patches) *n_patches = np;
   if (n_slots) *n_slots = int64_t(P.vert_ids.size());
   if (max_patch_verts) *max_patch_verts = mv;

@@ -22,6 +22,8 @@ hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, in
                                 int bs, hipStream_t st);
 hipError_t launch_patch_residual(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
                                  hipStream_t st);
+hipError_t launch_patch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                              hipStream_t st);
 hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st);
 hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                         hipStream_t st);

@@ -192,7 +192,7 @@ __device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v)
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int b = S.vslot_off[v], e = S.vslot_off[v + 1];
   for (int a = b; a < e; ++a) {
-    const float4 c = S.slot_acc[S.vslot[a]];
+    const float4 c = S.slot_acc[a];
     acc.x += c.x;
     acc.y += c.y;
     acc.z += c.z;
@@ -325,7 +325,9 @@ __global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* 
 
 // One workgroup per patch: stage the patch's vertices in LDS, project every
 // tet, accumulate the corrections per patch vertex in LDS, then write one
-// partial per (patch, vertex) slot.
+// partial per (patch, vertex) slot.  In deterministic mode each tet corner is
+// stored at its vertex's rank position in an LDS array sorted by vertex (ranks
+// follow tet order), and each vertex sums its contiguous run in that order.
 __global__ void __launch_bounds__(kPatchThreads)
     k_patch_project(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map,
                     int first) {
@@ -336,33 +338,45 @@ __global__ void __launch_bounds__(kPatchThreads)
   const int vb = S.p_vert_begin[p], nv = S.p_vert_begin[p + 1] - vb;
   __shared__ float4 s_y[kPatchVerts];
   __shared__ float4 s_rest[kPatchVerts];
-  __shared__ float4 s_acc[kPatchVerts];
-  __shared__ float4 s_corner[4 * kPatchThreads];
+  __shared__ int s_base[kPatchVerts];
+  // corner corrections sorted by vertex (deterministic) or per-vertex LDS atomics
+  __shared__ float4 s_sorted[4 * kPatchThreads];
+  float4* s_acc = s_sorted;
+  const bool csr = S.csr != 0;
+  const int lt = threadIdx.x;
+  // per-tet inputs are independent of the staging: issue them first
+  uint2 loc = make_uint2(0u, 0u);
+  float2 lam = make_float2(0.f, 0.f);
+  if (lt < nt) {
+    loc = S.tet_local[tb + lt];
+    if (!first) lam = S.lam[tb + lt];
+  }
+  int2 vsum = make_int2(0, 0);
   for (int i = threadIdx.x; i < nv; i += blockDim.x) {
     const int g = S.p_vert_ids[vb + i];
+    vsum = S.p_vert_sum[vb + i];
     s_y[i] = S.pos[g];
     s_rest[i] = S.rest[g];
-    s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    s_base[i] = vsum.x & 0xffff;
+    if (!csr) s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   __syncthreads();
-  const int lt = threadIdx.x;
   if (lt < nt) {
-    const int t = tb + lt;
-    const unsigned loc = S.tet_local[t];
-    const int l[4] = {int(loc & 0xffu), int((loc >> 8) & 0xffu), int((loc >> 16) & 0xffu),
-                      int(loc >> 24)};
+    const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu), int((loc.x >> 16) & 0xffu),
+                      int(loc.x >> 24)};
     const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
     Mat3 idm;
     float V0;
     rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
-    float2 lam = first ? make_float2(0.f, 0.f) : S.lam[t];
     TetProjection P;
     project_tet(S, q, idm, V0, lam, P);
-    S.lam[t] = lam;
-    if (S.csr) {
+    S.lam[tb + lt] = lam;
+    if (csr) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k)
-        s_corner[4 * lt + k] = make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
+      for (int k = 0; k < 4; ++k) {
+        const int rank = int((loc.y >> (8 * k)) & 0xffu);
+        s_sorted[s_base[l[k]] + rank] = make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
+      }
     } else {
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -374,13 +388,15 @@ __global__ void __launch_bounds__(kPatchThreads)
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
+  // nv <= blockDim.x: thread i still holds vertex i's record in vsum
+  if (threadIdx.x < nv) {
+    const int i = threadIdx.x;
     float4 a;
-    if (S.csr) {
+    if (csr) {
       a = make_float4(0.f, 0.f, 0.f, 0.f);
-      const int cb = S.lc_off[vb + i], ce = S.lc_off[vb + i + 1];
+      const int cb = vsum.x & 0xffff, ce = cb + (vsum.x >> 16);
       for (int c = cb; c < ce; ++c) {
-        const float4 v = s_corner[S.lc[c]];
+        const float4 v = s_sorted[c];
         a.x += v.x;
         a.y += v.y;
         a.z += v.z;
@@ -389,7 +405,7 @@ __global__ void __launch_bounds__(kPatchThreads)
     } else {
       a = s_acc[i];
     }
-    S.slot_acc[vb + i] = a;
+    S.slot_acc[vsum.y] = a;
   }
 }
 
@@ -415,9 +431,9 @@ __global__ void __launch_bounds__(kPatchThreads)
   const int p = wi.first;
   const int tb = S.p_tet_begin[p], nt = S.p_tet_begin[p + 1] - tb;
   const int vb = S.p_vert_begin[p], nv = S.p_vert_begin[p + 1] - vb;
-  const float4* X = S.cheb ? S.xj : S.pos;
   __shared__ float4 s_x[kPatchVerts];
   __shared__ float4 s_rest[kPatchVerts];
+  const float4* X = S.cheb ? S.xj : S.pos;
   for (int i = threadIdx.x; i < nv; i += blockDim.x) {
     const int g = S.p_vert_ids[vb + i];
     s_x[i] = X[g];
@@ -428,7 +444,7 @@ __global__ void __launch_bounds__(kPatchThreads)
   const int lt = threadIdx.x;
   if (lt < nt) {
     const int t = tb + lt;
-    const unsigned loc = S.tet_local[t];
+    const unsigned loc = S.tet_local[t].x;
     const int l[4] = {int(loc & 0xffu), int((loc >> 8) & 0xffu), int((loc >> 16) & 0xffu),
                       int(loc >> 24)};
     Mat3 idm;
@@ -445,14 +461,30 @@ __global__ void __launch_bounds__(kPatchThreads)
 // One workgroup per scene: fixed-order sum of the scene's partials, then the
 // residual-driven rho estimate and Chebyshev omega of solve_step
 // (neohookean_solver.py:492-504, 537-546) in double.
-__global__ void k_reduce(const SceneDev* __restrict__ scenes, int it) {
+__global__ void __launch_bounds__(1024) k_reduce(const SceneDev* __restrict__ scenes, int it) {
   const SceneDev& S = scenes[blockIdx.x];
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  for (int b = threadIdx.x; b < S.n_partials; b += blockDim.x) {
-    const double* part = S.partial + 3 * size_t(b);
-    s0 += part[0];
-    s1 += part[1];
-    s2 += part[2];
+  const int n = S.n_partials;
+  const double* part = S.partial;
+  int b = threadIdx.x;
+  // four independent loads in flight per component and step
+  for (; b + 3 * int(blockDim.x) < n; b += 4 * blockDim.x) {
+    double a[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a[u][c] = part[3 * size_t(b + u * blockDim.x) + c];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      s0 += a[u][0];
+      s1 += a[u][1];
+      s2 += a[u][2];
+    }
+  }
+  for (; b < n; b += blockDim.x) {
+    s0 += part[3 * size_t(b)];
+    s1 += part[3 * size_t(b) + 1];
+    s2 += part[3 * size_t(b) + 2];
   }
   s0 = wave_sum_d(s0);
   s1 = wave_sum_d(s1);
@@ -520,6 +552,24 @@ __global__ void k_blend(const SceneDev* __restrict__ scenes, const WorkItem* __r
   const float gamma = S.gamma;
   const float4 y4 = S.pos[v];
   const float3 y = f3(y4), xj = f3(S.xj[v]), pr = f3(S.prev[v]);
+  const float3 x_new = gamma * (xj - y) + y;
+  const float3 nw = pr + omega * (x_new - pr);
+  S.prev[v] = make_float4(y.x, y.y, y.z, 0.f);
+  S.pos[v] = make_float4(nw.x, nw.y, nw.z, y4.w);
+}
+
+// Chebyshev blend with x_J formed from the patch slots (patch layout).
+__global__ void k_patch_blend(const SceneDev* __restrict__ scenes,
+                              const WorkItem* __restrict__ map) {
+  const WorkItem wi = map[blockIdx.x];
+  const SceneDev& S = scenes[wi.scene];
+  const int v = wi.first + threadIdx.x;
+  if (v >= S.n_verts) return;
+  const float omega = float(S.ctrl->omega);
+  const float gamma = S.gamma;
+  const float4 xj4 = jacobi_average_slots(S, v);
+  const float4 y4 = S.pos[v];
+  const float3 y = f3(y4), xj = f3(xj4), pr = f3(S.prev[v]);
   const float3 x_new = gamma * (xj - y) + y;
   const float3 nw = pr + omega * (x_new - pr);
   S.prev[v] = make_float4(y.x, y.y, y.z, 0.f);
@@ -612,12 +662,17 @@ hipError_t launch_patch_residual(const SceneDev* scenes, const WorkItem* pmap, i
   return hipGetLastError();
 }
 hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st) {
-  k_reduce<<<n_scenes, 256, 0, st>>>(scenes, it);
+  k_reduce<<<n_scenes, 1024, 0, st>>>(scenes, it);
   return hipGetLastError();
 }
 hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                         hipStream_t st) {
   k_blend<<<nblocks, bs, 0, st>>>(scenes, vmap);
+  return hipGetLastError();
+}
+hipError_t launch_patch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                              hipStream_t st) {
+  k_patch_blend<<<nblocks, bs, 0, st>>>(scenes, vmap);
   return hipGetLastError();
 }
 hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,

@@ -10,7 +10,8 @@ reference evaluates in host Python (:492-546) runs on the device, so a frame
 needs no host round trip.
 
 Execution options with no reference counterpart are keyword-only:
-  accumulate_mode  "atomic" (float atomics) or "csr" (deterministic gather)
+  accumulate_mode  "csr" (deterministic fixed-order sums, default) or "atomic"
+                   (float atomics, the reference's unordered accumulation)
   reorder          Morton-reorder tets and vertices internally (default True)
   device           HIP device ordinal
   exec_path        "graph" (hipGraph per substep) or "kernels"
@@ -72,7 +73,7 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
                  block_neohookean_enable=False,
                  monitor_convergence=False,
                  *,
-                 accumulate_mode="atomic",
+                 accumulate_mode="csr",
                  reorder=True,
                  device=0,
                  exec_path="graph",
