@@ -74,7 +74,22 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-frames", type=int, default=1)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration_mib:
+        # A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings
+        # give the scale between those counters and bytes moved.
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 4, "launches": 3}), flush=True)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -116,7 +131,7 @@ def main():
     ms, launches = solver.kernel_time()
     solver.set_kernel_timing(False)
     avg_s = ms / 1e3 / max(1, launches)
-    ach = b_alg_project(T, V) / avg_s / 1e9
+    ach = b_alg_project(T, V) / avg_s / 1e9 if avg_s > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
                 "kernel": "k_project", "avg_us": round(avg_s * 1e6, 2), "launches": launches,

@@ -194,9 +194,13 @@ int xpbd_check_finite(xpbd_solver* s);
 
 /* Host-only description of the patch layout a mesh would get (no device
  * work): number of patches, number of (patch, vertex) slots, and the largest
- * vertex count of a patch.  tets: n_tets*4 ids in the order they are given. */
-int xpbd_patch_stats(const int32_t* tets, int32_t n_tets, int32_t n_verts, int32_t* n_patches,
-                     int64_t* n_slots, int32_t* max_patch_verts, int32_t* max_patch_tets);
+ * vertex and tet counts of a patch.  With positions (n_verts*3) the tets are
+ * ordered as the solver orders them (coordinate bisection); with NULL they
+ * are cut greedily in the order given.  The layout's invariants are checked
+ * and a violation is reported as XPBD_ESTATE. */
+int xpbd_patch_stats(const float* positions, const int32_t* tets, int32_t n_tets,
+                     int32_t n_verts, int32_t* n_patches, int64_t* n_slots,
+                     int32_t* max_patch_verts, int32_t* max_patch_tets);
 
 /* Copies the last error message of the calling thread into buf. */
 int xpbd_last_error(char* buf, size_t len);

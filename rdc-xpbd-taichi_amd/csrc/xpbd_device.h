@@ -68,7 +68,9 @@ struct SceneDev {
   const int* p_tet_begin;   // n_patches + 1
   const int* p_vert_begin;  // n_patches + 1: range of the patch's vertex records
   const int* p_vert_ids;    // per patch vertex: global vertex id
-  const int2* p_vert_sum;   // per patch vertex: (corner base | corner count << 16, slot)
+  const int2* p_vert_sum;   // per patch vertex: (corner base | corner count << 16 |
+                            //   owner << 31, slot); the owner patch writes the vertex
+  float4* ybuf[3];          // rotating Chebyshev iterates y^k (ybuf[0] is pos)
   const uint2* tet_local;   // per tet: 4 x u8 local vertex ids, 4 x u8 corner ranks
   const int* vslot_off;     // n_verts + 1: each vertex's slots, contiguous, ascending patch
   float4* slot_acc;         // per slot: partial delta_x (xyz) and constraint count (w)
@@ -82,6 +84,7 @@ struct SceneDev {
   int tet_block;    // flat layout: threads per tet workgroup
   int n_patches;
   int layout;       // 0 flat, 1 patch
+  int fused;        // patch layout with Chebyshev: blend and average folded into the tet passes
   // parameters as the reference kernels see them (f32 constants / arguments)
   float lame1, lame2;
   float dt;
@@ -106,7 +109,26 @@ struct WorkItem {
   int first;
 };
 
+// One patch of one scene, as walked by the persistent patch kernels.
+struct __align__(32) PatchItem {
+  int scene;
+  int patch;
+  int tet_begin;
+  int vert_begin;
+  int n_tets;
+  int n_verts;
+  int pad[2];
+};
+
 constexpr float kEps = 1e-12f;  // xpbd_gpu/constants.py:3 as an f32 constant
+
+// Hot-path division and square root on the native gfx950 instructions
+// (v_rcp_f32, v_sqrt_f32: about 1 ulp) instead of the correctly rounded
+// sequences (a dozen instructions each).  The reference's f32 results are
+// reproduced to the same round-off tolerance as with exact division.
+__device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 __device__ __forceinline__ float3 f3(const float4& a) { return make_float3(a.x, a.y, a.z); }
 __device__ __forceinline__ float3 operator+(float3 a, float3 b) {
@@ -215,21 +237,22 @@ __device__ __forceinline__ void solve2x2_with_fallback(float A11, float A12, flo
   x0 = 0.f;
   x1 = 0.f;
   if (ok) {
-    const float m = a10 / a00;
+    const float inv00 = frcp(a00);
+    const float m = a10 * inv00;
     const float a11_ = a11 - m * a01;
     const float c1_ = c1 - m * c0;
     if (fabsf(a11_) < kEps) {
       ok = 0;
     } else {
-      x1 = c1_ / a11_;
-      x0 = (c0 - a01 * x1) / a00;
+      x1 = fdiv(c1_, a11_);
+      x0 = (c0 - a01 * x1) * inv00;
     }
   }
   if (!ok) {
     x0 = 0.f;
     x1 = 0.f;
-    if (fabsf(A11) > kEps) x0 = b1 / A11;
-    if (fabsf(A22) > kEps) x1 = b2 / A22;
+    if (fabsf(A11) > kEps) x0 = fdiv(b1, A11);
+    if (fabsf(A22) > kEps) x1 = fdiv(b2, A22);
   }
 }
 
@@ -262,21 +285,21 @@ __device__ __forceinline__ float3 sdf_project(const SdfDev& s, float3 x, float3 
   return x;
 }
 
-// V0 and inv_Dm of a tet from its rest positions, evaluated exactly as the
-// set-up does (calc_tet_volume, xpbd_base.py:6-8; Matrix.inverse of
-// cols(x1-x0, x2-x0, x3-x0), neohookean_solver.py:102-107) with no
-// contraction, so every bit matches the values the flat layout stores.
+// V0 and inv_Dm of a tet from its rest positions, the formulas of the set-up
+// (calc_tet_volume, xpbd_base.py:6-8; Matrix.inverse of cols(x1-x0, x2-x0,
+// x3-x0), neohookean_solver.py:102-107), recomputed per use instead of
+// streaming 40 bytes per tet; they agree with the stored f32 values to a few
+// ulp.
 __device__ __forceinline__ void rest_shape(float3 p0, float3 p1, float3 p2, float3 p3, Mat3& inv,
                                            float& V0) {
-#pragma clang fp contract(off)
   const float3 a = p1 - p0, b = p2 - p0, c = p3 - p0;
   const float3 ab = make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-  V0 = (ab.x * c.x + ab.y * c.y + ab.z * c.z) / 6.f;
+  V0 = (ab.x * c.x + ab.y * c.y + ab.z * c.z) * (1.f / 6.f);
   const float m[3][3] = {{a.x, b.x, c.x}, {a.y, b.y, c.y}, {a.z, b.z, c.z}};
   const float det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
                     m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
                     m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
-  const float inv_det = 1.f / det;
+  const float inv_det = frcp(det);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll

@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <numeric>
@@ -29,6 +30,7 @@ using xpbd::SceneCtrl;
 using xpbd::SceneDev;
 using xpbd::SdfDev;
 using xpbd::WorkItem;
+using xpbd::PatchItem;
 
 namespace {
 
@@ -168,12 +170,12 @@ int validate_sdfs(const xpbd_sdf* sdfs, int n) {
   return XPBD_OK;
 }
 
-// Patches: consecutive (Morton-ordered) tets grouped into workgroup-sized
-// pieces of at most kPatchTets tets touching at most kPatchVerts vertices, so
-// that local vertex ids fit in a byte.  Each (patch, vertex) pair is a slot;
+// Patches: consecutive tets grouped into workgroup-sized pieces of at most
+// kPatchTets tets touching at most kPatchVerts vertices (local ids and corner
+// ranks fit in a byte).  Each (patch, vertex) pair is a slot;
 // a vertex's slots are listed in ascending patch order.
-constexpr int kPatchTets = 256;
-constexpr int kPatchVerts = 256;
+constexpr int kPatchTets = 256;   // one tet per thread of a 256-thread workgroup
+constexpr int kPatchVerts = 256;  // one vertex per thread; u8 local ids
 
 struct Patches {
   std::vector<int> tet_begin, vert_begin, vert_ids, vslot_off;
@@ -181,7 +183,8 @@ struct Patches {
   std::vector<uint2> tet_local;   // (4 x u8 local ids, 4 x u8 corner ranks)
 };
 
-void build_patches(const std::vector<int>& tets, int nv, int nt, Patches& P) {
+void build_patches(const std::vector<int>& tets, int nv, int nt, Patches& P,
+                   const std::vector<int>* cuts = nullptr) {
   P.tet_begin.assign(1, 0);
   P.vert_begin.assign(1, 0);
   P.vert_ids.clear();
@@ -216,8 +219,13 @@ void build_patches(const std::vector<int>& tets, int nv, int nt, Patches& P) {
     cur.clear();
     start = end;
   };
+  size_t next_cut = 0;
   for (int t = 0; t < nt; ++t) {
     const int p = int(P.tet_begin.size()) - 1;
+    if (cuts && next_cut < cuts->size() && t == (*cuts)[next_cut] && t > start) {
+      close(t);
+      ++next_cut;
+    }
     int fresh = 0, crowded = 0;
     for (int k = 0; k < 4; ++k) {
       const int g = tets[4 * size_t(t) + k];
@@ -242,7 +250,76 @@ void build_patches(const std::vector<int>& tets, int nv, int nt, Patches& P) {
   for (int g : P.vert_ids) P.vslot_off[g + 1]++;
   for (int v = 0; v < nv; ++v) P.vslot_off[v + 1] += P.vslot_off[v];
   std::vector<int> fill(P.vslot_off.begin(), P.vslot_off.end() - 1);
-  for (size_t i = 0; i < P.vert_ids.size(); ++i) P.vert_sum[i].y = fill[P.vert_ids[i]]++;
+  for (size_t i = 0; i < P.vert_ids.size(); ++i) {
+    const int g = P.vert_ids[i];
+    P.vert_sum[i].y = fill[g]++;
+    // the patch holding a vertex's first slot owns the vertex's writes
+    if (P.vert_sum[i].y == P.vslot_off[g]) P.vert_sum[i].x |= int(0x80000000u);
+  }
+}
+
+// Recursive coordinate bisection of the tets by centroid: the tet set is cut
+// at a multiple of kPatchTets along the longest axis of its centroid box
+// until every piece has at most kPatchTets tets and kPatchVerts vertices (and
+// no vertex in more than 255 of its tets).  The pieces are compact blobs, so
+// a patch stages few vertices per tet; `order` lists the tets piece by piece
+// and `cuts` the end of each piece.
+void rcb_order(const std::vector<float>& cent, const int* tets, int nt, int nv,
+               std::vector<int>& order, std::vector<int>& cuts) {
+  order.resize(nt);
+  std::iota(order.begin(), order.end(), 0);
+  cuts.clear();
+  std::vector<int> stamp(nv, -1), lcount(nv, 0);
+  int stamp_id = 0;
+  auto leaf_ok = [&](int lo, int hi) {
+    ++stamp_id;
+    int nverts = 0;
+    for (int i = lo; i < hi; ++i)
+      for (int k = 0; k < 4; ++k) {
+        const int g = tets[4 * size_t(order[i]) + k];
+        if (stamp[g] != stamp_id) {
+          stamp[g] = stamp_id;
+          lcount[g] = 0;
+          ++nverts;
+        }
+        if (++lcount[g] > 255) return false;
+      }
+    return nverts <= kPatchVerts;
+  };
+  std::vector<int2> stack;
+  if (nt > 0) stack.push_back(make_int2(0, nt));
+  std::vector<int2> leaves;
+  while (!stack.empty()) {
+    const int2 r = stack.back();
+    stack.pop_back();
+    const int lo = r.x, hi = r.y, n = hi - lo;
+    if (n <= kPatchTets && leaf_ok(lo, hi)) {
+      leaves.push_back(r);
+      continue;
+    }
+    float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = lo; i < hi; ++i)
+      for (int k = 0; k < 3; ++k) {
+        mn[k] = std::min(mn[k], cent[3 * size_t(order[i]) + k]);
+        mx[k] = std::max(mx[k], cent[3 * size_t(order[i]) + k]);
+      }
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+      if (mx[k] - mn[k] > mx[axis] - mn[axis]) axis = k;
+    int m;
+    if (n > kPatchTets) {
+      const int pieces = (n + kPatchTets - 1) / kPatchTets;
+      m = lo + (pieces / 2) * kPatchTets;
+    } else {
+      m = lo + n / 2;
+    }
+    std::nth_element(order.begin() + lo, order.begin() + m, order.begin() + hi,
+                     [&](int a, int b) { return cent[3 * size_t(a) + axis] < cent[3 * size_t(b) + axis]; });
+    // depth first, low half first: push high then low
+    stack.push_back(make_int2(m, hi));
+    stack.push_back(make_int2(lo, m));
+  }
+  for (const int2& l : leaves) cuts.push_back(l.y);
 }
 
 }  // namespace
@@ -259,11 +336,14 @@ struct xpbd_solver {
   std::vector<int> tets_new;       // internal order, internal vertex ids
   std::vector<float> m_host, inv_m_host;  // internal order
 
-  DevBuf<float4> pos, xs, vel, prev, xj, dacc, corner, rest, slot_acc;
+  DevBuf<float4> pos, xs, vel, prev, xj, dacc, corner, rest, slot_acc, ybuf1, ybuf2;
+  bool has_isolated = false;
   DevBuf<int> p_tet_begin, p_vert_begin, p_vert_ids, vslot_off;
   DevBuf<int2> p_vert_sum;
   DevBuf<uint2> tet_local;
-  DevBuf<WorkItem> pmap;
+  DevBuf<xpbd::PatchItem> pitems;
+  DevBuf<int2> prange_proj, prange_res;
+  int grid_proj = 0, grid_res = 0;
   int n_patches = 0;
   int layout = XPBD_LAYOUT_PATCH;
   DevBuf<float> valence;
@@ -346,6 +426,7 @@ void fill_scene_params(xpbd_solver* s, SceneDev& S, double dt, double damping) {
   S.cheb = d.cheb_enable ? 1 : 0;
   S.monitor = d.monitor_convergence ? 1 : 0;
   S.csr = d.accumulate_mode == XPBD_ACCUM_CSR ? 1 : 0;
+  S.fused = (s->layout == XPBD_LAYOUT_PATCH && S.cheb && !s->has_isolated) ? 1 : 0;
   S.warmup = d.cheb_warmup;
   S.iters = d.constraints_iter;
   S.n_sdf = int(s->sdfs.size());
@@ -374,6 +455,9 @@ int build_scene(xpbd_solver* s) {
   S.p_vert_begin = s->p_vert_begin.p;
   S.p_vert_ids = s->p_vert_ids.p;
   S.p_vert_sum = s->p_vert_sum.p;
+  S.ybuf[0] = s->pos.p;
+  S.ybuf[1] = s->ybuf1.p;
+  S.ybuf[2] = s->ybuf2.p;
   S.tet_local = s->tet_local.p;
   S.vslot_off = s->vslot_off.p;
   S.slot_acc = s->slot_acc.p;
@@ -404,10 +488,67 @@ int sync_table(xpbd_solver* s, double dt, double damping) {
   return XPBD_OK;
 }
 
+// Workgroups are dispatched round-robin over the accelerator dies (XCDs), each
+// with its own L2: workgroup b runs on XCD b % n_xcd.  This returns, for each
+// workgroup, the work item it should take so that every XCD walks one
+// contiguous range of the (Morton-ordered) items, and neighbouring patches and
+// their vertices meet in the same L2.
+std::vector<int> xcd_order(int n, int n_xcd) {
+  std::vector<int> order(n);
+  if (n_xcd <= 1) {
+    std::iota(order.begin(), order.end(), 0);
+    return order;
+  }
+  std::vector<int> start(n_xcd + 1, 0);
+  for (int x = 0; x < n_xcd; ++x) start[x + 1] = start[x] + (n > x ? (n - x - 1) / n_xcd + 1 : 0);
+  for (int b = 0; b < n; ++b) order[b] = start[b % n_xcd] + b / n_xcd;
+  return order;
+}
+
+int xcd_count() {
+  const char* e = std::getenv("XPBD_XCDS");
+  const int v = e ? std::atoi(e) : 8;  // MI355X: 8 XCDs of 32 CUs
+  return v >= 1 ? v : 1;
+}
+
+// Splits n_items work items over n_blocks persistent workgroups: each XCD gets
+// a contiguous share proportional to its number of workgroups, and each
+// workgroup a contiguous run within its XCD's share.  Every item is covered
+// exactly once (checked).
+int xcd_ranges(int n_items, int n_blocks, int n_xcd, std::vector<int2>& ranges) {
+  ranges.assign(n_blocks, make_int2(0, 0));
+  if (n_blocks <= 0) return XPBD_OK;
+  n_xcd = std::max(1, std::min(n_xcd, n_blocks));
+  std::vector<int> nb(n_xcd, 0);
+  for (int b = 0; b < n_blocks; ++b) nb[b % n_xcd]++;
+  long long before = 0;
+  for (int x = 0; x < n_xcd; ++x) {
+    const int is = int(before * n_items / n_blocks);
+    before += nb[x];
+    const int ie = int(before * n_items / n_blocks);
+    for (int j = 0; j < nb[x]; ++j) {
+      const int b = x + j * n_xcd;
+      ranges[b] = make_int2(is + int((long long)j * (ie - is) / nb[x]),
+                            is + int((long long)(j + 1) * (ie - is) / nb[x]));
+    }
+  }
+  std::vector<char> seen(n_items, 0);
+  for (const int2& r : ranges)
+    for (int i = r.x; i < r.y; ++i) {
+      if (i < 0 || i >= n_items || seen[i]) return set_error(XPBD_ESTATE, "work split overlaps");
+      seen[i] = 1;
+    }
+  for (char c : seen)
+    if (!c) return set_error(XPBD_ESTATE, "work split misses an item");
+  return XPBD_OK;
+}
+
 int build_maps(xpbd_solver* s) {
   std::vector<WorkItem> vm, tm;
-  for (int v = 0; v < s->nv; v += s->bs_v) vm.push_back({0, v});
-  for (int t = 0; t < s->nt; t += s->bs_t) tm.push_back({0, t});
+  const int nx = xcd_count();
+  const int nvb = (s->nv + s->bs_v - 1) / s->bs_v, ntb = (s->nt + s->bs_t - 1) / s->bs_t;
+  for (int b : xcd_order(nvb, nx)) vm.push_back({0, b * s->bs_v});
+  for (int b : xcd_order(ntb, nx)) tm.push_back({0, b * s->bs_t});
   s->nvb = int(vm.size());
   s->ntb = int(tm.size());
   TRY(s->vmap.upload(vm.data(), vm.size()));
@@ -504,6 +645,11 @@ int events_collect(xpbd_solver* s) {
 }
 
 // The Jacobi iterations of solve_step (neohookean_solver.py:474-549).
+//  fused (patch layout, Chebyshev on): per iteration a projection (which
+//    first forms the previous iteration's Chebyshev blend), the Jacobi
+//    average, a residual and the scalar rho/omega update; a last vertex pass
+//    forms the final blend.
+//  otherwise: projection, Jacobi average, [residual, rho/omega], [blend].
 int enqueue_iterations(xpbd_solver* s, bool capturing) {
   const SceneDev& S = s->host_scene;
   const SceneDev* T = s->table.p;
@@ -513,24 +659,27 @@ int enqueue_iterations(xpbd_solver* s, bool capturing) {
     const bool timed = s->timing && !capturing;
     if (timed) TRY(events_begin(s, ev));
     if (patch)
-      HIP_TRY(xpbd::launch_patch_project(T, s->pmap.p, s->n_patches, it == 0, s->stream));
+      HIP_TRY(xpbd::launch_patch_project(T, s->pitems.p, s->prange_proj.p, s->grid_proj, it,
+                                         s->stream));
     else
       HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
     if (timed) TRY(events_end(s, ev));
-    if (patch) {
-      HIP_TRY(xpbd::launch_patch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
-    } else {
+    if (patch)
+      HIP_TRY(xpbd::launch_patch_average(T, s->vmap.p, s->nvb, s->bs_v, it, s->stream));
+    else
       HIP_TRY(xpbd::launch_average(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
-    }
     if (S.cheb || S.monitor) {
       if (patch)
-        HIP_TRY(xpbd::launch_patch_residual(T, s->pmap.p, s->n_patches, s->stream));
+        HIP_TRY(xpbd::launch_patch_residual(T, s->pitems.p, s->prange_res.p, s->grid_res,
+                                            s->stream));
       else
         HIP_TRY(xpbd::launch_residual(T, s->tmap.p, s->ntb, s->bs_t, s->stream));
       HIP_TRY(xpbd::launch_reduce(T, 1, it, s->stream));
     }
-    if (S.cheb) HIP_TRY(xpbd::launch_blend(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
+    if (S.cheb && !S.fused) HIP_TRY(xpbd::launch_blend(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
   }
+  if (S.fused && S.iters > 0)
+    HIP_TRY(xpbd::launch_final_blend(T, s->vmap.p, s->nvb, s->bs_v, S.iters, s->stream));
   return XPBD_OK;
 }
 
@@ -550,7 +699,7 @@ int enqueue_substep(xpbd_solver* s, bool capturing) {
 }
 
 int graph_key_of(const SceneDev& S) {
-  return (S.iters << 5) | (S.cheb << 0) | (S.monitor << 1) | (S.block_mode << 2) | (S.csr << 3) |
+  return (S.iters << 6) | (S.fused << 5) | (S.cheb << 0) | (S.monitor << 1) | (S.block_mode << 2) | (S.csr << 3) |
          (S.layout << 4);
 }
 
@@ -711,6 +860,31 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
     std::stable_sort(s->t_new2old.begin(), s->t_new2old.end(),
                      [&](int a, int b) { return tc[a] < tc[b]; });
   }
+  std::vector<int> patch_cuts;
+  if (d->reorder && d->layout == XPBD_LAYOUT_PATCH) {
+    // compact patches by coordinate bisection; vertices numbered by their first
+    // appearance in patch order, so a patch's vertices are nearly contiguous
+    std::vector<float> cent(3 * size_t(nt));
+    for (int t = 0; t < nt; ++t)
+      for (int j = 0; j < 3; ++j) {
+        float c = 0.f;
+        for (int k = 0; k < 4; ++k) c += 0.25f * x0[3 * size_t(d->tets[4 * size_t(t) + k]) + j];
+        cent[3 * size_t(t) + j] = c;
+      }
+    rcb_order(cent, d->tets, nt, nv, s->t_new2old, patch_cuts);
+    std::vector<int> seen(nv, 0);
+    s->v_new2old.clear();
+    for (int t = 0; t < nt; ++t)
+      for (int k = 0; k < 4; ++k) {
+        const int g = d->tets[4 * size_t(s->t_new2old[t]) + k];
+        if (!seen[g]) {
+          seen[g] = 1;
+          s->v_new2old.push_back(g);
+        }
+      }
+    for (int g = 0; g < nv; ++g)
+      if (!seen[g]) s->v_new2old.push_back(g);
+  }
   s->v_old2new.resize(nv);
   for (int i = 0; i < nv; ++i) s->v_old2new[s->v_new2old[i]] = i;
   s->tets_orig.assign(d->tets, d->tets + 4 * size_t(nt));
@@ -776,13 +950,15 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
   }
   s->layout = d->layout;
   if (d->layout == XPBD_LAYOUT_FLAT) {
-    TRY(s->tet.upload(h_tet.data(), nt));
     TRY(s->dm0.upload(h_dm0.data(), nt));
     TRY(s->dm1.upload(h_dm1.data(), nt));
     TRY(s->dm2.upload(h_dm2.data(), nt));
+  }
+  if (d->layout == XPBD_LAYOUT_FLAT) {
+    TRY(s->tet.upload(h_tet.data(), nt));
   } else {
     Patches P;
-    build_patches(s->tets_new, nv, nt, P);
+    build_patches(s->tets_new, nv, nt, P, patch_cuts.empty() ? nullptr : &patch_cuts);
     s->n_patches = int(P.tet_begin.size()) - 1;
     TRY(s->p_tet_begin.upload(P.tet_begin.data(), P.tet_begin.size()));
     TRY(s->p_vert_begin.upload(P.vert_begin.data(), P.vert_begin.size()));
@@ -792,9 +968,32 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
     TRY(s->vslot_off.upload(P.vslot_off.data(), P.vslot_off.size()));
     TRY(s->slot_acc.alloc(P.vert_ids.size()));
     TRY(s->slot_acc.zero());
-    std::vector<WorkItem> pm(s->n_patches);
-    for (int p = 0; p < s->n_patches; ++p) pm[p] = {0, p};
-    TRY(s->pmap.upload(pm.data(), pm.size()));
+    TRY(s->ybuf1.alloc(nv));
+    TRY(s->ybuf2.alloc(nv));
+    for (int i = 0; i < nv; ++i)
+      if (val[i] == 0.f) s->has_isolated = true;
+    std::vector<PatchItem> items(s->n_patches);
+    for (int p = 0; p < s->n_patches; ++p) {
+      PatchItem& it = items[p];
+      std::memset(&it, 0, sizeof(it));
+      it.scene = 0;
+      it.patch = p;
+      it.tet_begin = P.tet_begin[p];
+      it.vert_begin = P.vert_begin[p];
+      it.n_tets = P.tet_begin[p + 1] - P.tet_begin[p];
+      it.n_verts = P.vert_begin[p + 1] - P.vert_begin[p];
+    }
+    TRY(s->pitems.upload(items.data(), items.size()));
+    int occ_p = 0, occ_r = 0, n_cu = 0;
+    HIP_TRY(xpbd::patch_kernel_occupancy(&occ_p, &occ_r));
+    HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, d->device));
+    s->grid_proj = std::max(1, std::min(s->n_patches, n_cu * std::max(1, occ_p)));
+    s->grid_res = std::max(1, std::min(s->n_patches, n_cu * std::max(1, occ_r)));
+    std::vector<int2> rp, rr;
+    TRY(xcd_ranges(s->n_patches, s->grid_proj, xcd_count(), rp));
+    TRY(xcd_ranges(s->n_patches, s->grid_res, xcd_count(), rr));
+    TRY(s->prange_proj.upload(rp.data(), rp.size()));
+    TRY(s->prange_res.upload(rr.data(), rr.size()));
   }
   TRY(s->lam.alloc(nt));
   TRY(s->lam.zero());
@@ -1152,14 +1351,38 @@ int xpbd_get_kernel_time(xpbd_solver* s, double* total_ms, int64_t* launches) {
   return XPBD_OK;
 }
 
-int xpbd_patch_stats(const int32_t* tets, int32_t n_tets, int32_t n_verts, int32_t* n_patches,
-                     int64_t* n_slots, int32_t* max_patch_verts, int32_t* max_patch_tets) {
+int xpbd_patch_stats(const float* positions, const int32_t* tets, int32_t n_tets, int32_t n_verts,
+                     int32_t* n_patches, int64_t* n_slots, int32_t* max_patch_verts,
+                     int32_t* max_patch_tets) {
   if (!tets || n_tets < 0 || n_verts <= 0) return set_error(XPBD_EINVAL, "bad mesh");
   for (size_t i = 0; i < 4 * size_t(n_tets); ++i)
     if (tets[i] < 0 || tets[i] >= n_verts) return set_error(XPBD_EINVAL, "vertex id out of range");
   std::vector<int> t(tets, tets + 4 * size_t(n_tets));
+  std::vector<int> cuts;
+  if (positions) {  // the ordering the solver uses: bisection + first-appearance numbering
+    std::vector<float> cent(3 * size_t(n_tets));
+    for (int e = 0; e < n_tets; ++e)
+      for (int j = 0; j < 3; ++j) {
+        float c = 0.f;
+        for (int k = 0; k < 4; ++k) c += 0.25f * positions[3 * size_t(tets[4 * size_t(e) + k]) + j];
+        cent[3 * size_t(e) + j] = c;
+      }
+    std::vector<int> order;
+    rcb_order(cent, tets, n_tets, n_verts, order, cuts);
+    std::vector<int> newid(n_verts, -1);
+    int next = 0;
+    for (int e = 0; e < n_tets; ++e)
+      for (int k = 0; k < 4; ++k) {
+        const int g = tets[4 * size_t(order[e]) + k];
+        if (newid[g] < 0) newid[g] = next++;
+      }
+    for (int g = 0; g < n_verts; ++g)
+      if (newid[g] < 0) newid[g] = next++;
+    for (int e = 0; e < n_tets; ++e)
+      for (int k = 0; k < 4; ++k) t[4 * size_t(e) + k] = newid[tets[4 * size_t(order[e]) + k]];
+  }
   Patches P;
-  build_patches(t, n_verts, n_tets, P);
+  build_patches(t, n_verts, n_tets, P, cuts.empty() ? nullptr : &cuts);
   const int np = int(P.tet_begin.size()) - 1;
   int mv = 0, mt = 0;
   for (int p = 0; p < np; ++p) {
@@ -1185,7 +1408,7 @@ int xpbd_patch_stats(const int32_t* tets, int32_t n_tets, int32_t n_verts, int32
       for (int k = 0; k < 4; ++k) {
         const int l = int((P.tet_local[tt].x >> (8 * k)) & 0xffu);
         const int rank = int((P.tet_local[tt].y >> (8 * k)) & 0xffu);
-        const int base = P.vert_sum[vb + l].x & 0xffff, cnt = P.vert_sum[vb + l].x >> 16;
+        const int base = P.vert_sum[vb + l].x & 0xffff, cnt = (P.vert_sum[vb + l].x >> 16) & 0x7fff;
         const int pos = base + rank;
         if (rank >= cnt || pos >= 4 * ntp || seen[pos])
           return set_error(XPBD_ESTATE, "patch %d: corner position %d invalid or reused", p, pos);
@@ -1195,6 +1418,8 @@ int xpbd_patch_stats(const int32_t* tets, int32_t n_tets, int32_t n_verts, int32
       const int g = P.vert_ids[i], dst = P.vert_sum[i].y;
       if (dst < P.vslot_off[g] || dst >= P.vslot_off[g + 1] || seen_slot[dst])
         return set_error(XPBD_ESTATE, "slot %d of vertex %d out of its range or reused", dst, g);
+      if ((P.vert_sum[i].x < 0) != (dst == P.vslot_off[g]))
+        return set_error(XPBD_ESTATE, "vertex %d: owner flag not on its first slot", g);
       seen_slot[dst] = 1;
     }
   }

@@ -7,6 +7,7 @@ namespace xpbd {
 
 struct SceneDev;
 struct WorkItem;
+struct PatchItem;
 
 hipError_t launch_predict(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                           int external, int collide, int save_prev, hipStream_t st);
@@ -16,12 +17,15 @@ hipError_t launch_average(const SceneDev* scenes, const WorkItem* vmap, int nblo
                           hipStream_t st);
 hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nblocks, int bs,
                            hipStream_t st);
-hipError_t launch_patch_project(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
-                                int first, hipStream_t st);
+hipError_t launch_patch_project(const SceneDev* scenes, const PatchItem* items,
+                                const int2* ranges, int nblocks, int it, hipStream_t st);
 hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks,
-                                int bs, hipStream_t st);
-hipError_t launch_patch_residual(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
-                                 hipStream_t st);
+                                int bs, int it, hipStream_t st);
+hipError_t launch_patch_residual(const SceneDev* scenes, const PatchItem* items,
+                                 const int2* ranges, int nblocks, hipStream_t st);
+hipError_t patch_kernel_occupancy(int* project_blocks_per_cu, int* residual_blocks_per_cu);
+hipError_t launch_final_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                              int n_iter, hipStream_t st);
 hipError_t launch_patch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                               hipStream_t st);
 hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st);

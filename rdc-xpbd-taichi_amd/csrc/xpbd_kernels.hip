@@ -75,8 +75,9 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
                 w[3] * dot(gH[3], gH[3]);
     const float A12 = w[0] * dot(gD[0], gH[0]) + w[1] * dot(gD[1], gH[1]) +
                       w[2] * dot(gD[2], gH[2]) + w[3] * dot(gD[3], gH[3]);
-    const float aD = (1.f / (S.lame2 * V0)) / (dt * dt);
-    const float aH = (1.f / (S.lame1 * V0)) / (dt * dt);
+    const float inv_dt2 = frcp(dt * dt);
+    const float aD = frcp(S.lame2 * V0) * inv_dt2;
+    const float aH = frcp(S.lame1 * V0) * inv_dt2;
     A11 += aD;
     A22 += aH;
     const float b1 = -(C_D + aD * lam.x);
@@ -101,8 +102,8 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
     float dlH = 0.f;
     const int okH = wsH > kEps;
     if (okH) {
-      const float at = (1.f / (S.lame1 * V0)) / (dt * dt);
-      dlH = -(C_H + at * lam.y) / (wsH + at);
+      const float at = frcp(S.lame1 * V0) * frcp(dt * dt);
+      dlH = fdiv(-(C_H + at * lam.y), wsH + at);
       dlH *= S.hydro_relax;
       lam.y += dlH;
     }
@@ -120,8 +121,8 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
     float dlD = 0.f;
     const int okD = wsD > kEps;
     if (okD) {
-      const float at = (1.f / (S.lame2 * V0)) / (dt * dt);
-      dlD = -(C_D + at * lam.x) / (wsD + at);
+      const float at = frcp(S.lame2 * V0) * frcp(dt * dt);
+      dlD = fdiv(-(C_D + at * lam.x), wsD + at);
       dlD *= S.dev_relax;
       lam.x += dlD;
     }
@@ -147,11 +148,12 @@ __device__ __forceinline__ void residual_tet(const SceneDev& S, float3 x0, float
   r = 0.f;
   if (S.cheb) {
     const float dt = S.dt;
-    const float aD = (1.f / (S.lame2 * V0)) / (dt * dt);
-    const float aH = (1.f / (S.lame1 * V0)) / (dt * dt);
+    const float inv_dt2 = frcp(dt * dt);
+    const float aD = frcp(S.lame2 * V0) * inv_dt2;
+    const float aH = frcp(S.lame1 * V0) * inv_dt2;
     const float rD = C_D + aD * lam.x;
     const float rH = C_H + aH * lam.y;
-    r = sqrtf(rD * rD + rH * rH);
+    r = fsqrt(rD * rD + rH * rH);
   }
   ah = fabsf(C_H);
   ad = fabsf(C_D);
@@ -188,7 +190,7 @@ __device__ __forceinline__ void store_partial(const SceneDev& S, int idx, float 
 // x_J = y + delta_x / cons_num for one vertex (post_solve,
 // neohookean_solver.py:115-119), with delta_x summed from the vertex's patch
 // slots in ascending patch order.
-__device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v) {
+__device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v, int it) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   const int b = S.vslot_off[v], e = S.vslot_off[v + 1];
   for (int a = b; a < e; ++a) {
@@ -199,9 +201,12 @@ __device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v)
     acc.w += c.w;
   }
   const float cnt = S.block_mode ? S.valence[v] : acc.w;
-  const float4 p = S.pos[v];
+  const float4 p = S.fused ? S.ybuf[it % 3][v] : S.pos[v];
   float3 x = f3(p);
-  if (cnt > 0.f) x = x + make_float3(acc.x / cnt, acc.y / cnt, acc.z / cnt);
+  if (cnt > 0.f) {
+    const float ic = frcp(cnt);
+    x = x + make_float3(acc.x * ic, acc.y * ic, acc.z * ic);
+  }
   return make_float4(x.x, x.y, x.z, p.w);
 }
 
@@ -323,137 +328,285 @@ __global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* 
 
 // ---- patch layout ----------------------------------------------------------------
 
-// One workgroup per patch: stage the patch's vertices in LDS, project every
-// tet, accumulate the corrections per patch vertex in LDS, then write one
-// partial per (patch, vertex) slot.  In deterministic mode each tet corner is
-// stored at its vertex's rank position in an LDS array sorted by vertex (ranks
-// follow tet order), and each vertex sums its contiguous run in that order.
+// The Chebyshev blend of one vertex (_chebyshev_update + _rotate_prev_with_save,
+// neohookean_solver.py:131-142): y^{k+1} = prev + omega (gamma (x_J - y) + y - prev).
+__device__ __forceinline__ float4 cheb_blend(const SceneDev& S, float4 y4, float4 xj4, float4 pr4,
+                                             float omega) {
+  const float gamma = S.gamma;
+  const float3 y = f3(y4), xj = f3(xj4), pr = f3(pr4);
+  const float3 x_new = gamma * (xj - y) + y;
+  const float3 nw = pr + omega * (x_new - pr);
+  return make_float4(nw.x, nw.y, nw.z, y4.w);
+}
+
+// Iterate y^it of vertex g as staged by the projection of iteration it.  In
+// the fused sequence y^it is formed here from y^(it-1) (ybuf[(it-1)%3]),
+// x_J^(it-1) (xj), prev = y^(it-2) (or y^0 when it == 1) and omega^(it-1), and
+// the vertex's owner patch stores it in ybuf[it%3]; the other patches holding
+// the vertex compute the same bits.
+__device__ __forceinline__ float4 stage_iterate(const SceneDev& S, int g, int it, bool owner) {
+  if (!S.fused || it == 0) return S.pos[g];
+  const float4 y = S.ybuf[(it - 1) % 3][g];
+  const float4 pr = it == 1 ? y : S.ybuf[(it - 2) % 3][g];
+  const float4 out = cheb_blend(S, y, S.xj[g], pr, float(S.ctrl->omega));
+  if (owner) S.ybuf[it % 3][g] = out;
+  return out;
+}
+
+// Persistent patch projection.  Workgroup b walks the patch items
+// [ranges[b].x, ranges[b].y): it stages a patch's vertices and rest positions
+// in LDS, projects every tet, accumulates the corrections per patch vertex in
+// LDS, and writes one partial per (patch, vertex) slot.  The loads are
+// software-pipelined: while patch i is computed, the vertex data of patch i+1
+// and the vertex ids of patch i+2 are in flight.  Deterministic mode stores
+// each tet corner at its vertex's rank position in an LDS array sorted by
+// vertex (ranks follow tet order) and sums each vertex's run in that order;
+// atomic mode adds into per-vertex LDS cells.
 __global__ void __launch_bounds__(kPatchThreads)
-    k_patch_project(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map,
-                    int first) {
-  const WorkItem wi = map[blockIdx.x];
-  const SceneDev& S = scenes[wi.scene];
-  const int p = wi.first;
-  const int tb = S.p_tet_begin[p], nt = S.p_tet_begin[p + 1] - tb;
-  const int vb = S.p_vert_begin[p], nv = S.p_vert_begin[p + 1] - vb;
+    k_patch_project(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
+                    const int2* __restrict__ ranges, int it) {
+  const int2 rg = ranges[blockIdx.x];
+  if (rg.x >= rg.y) return;
+  const int first = it == 0;
   __shared__ float4 s_y[kPatchVerts];
   __shared__ float4 s_rest[kPatchVerts];
   __shared__ int s_base[kPatchVerts];
   // corner corrections sorted by vertex (deterministic) or per-vertex LDS atomics
   __shared__ float4 s_sorted[4 * kPatchThreads];
   float4* s_acc = s_sorted;
-  const bool csr = S.csr != 0;
-  const int lt = threadIdx.x;
-  // per-tet inputs are independent of the staging: issue them first
-  uint2 loc = make_uint2(0u, 0u);
-  float2 lam = make_float2(0.f, 0.f);
-  if (lt < nt) {
-    loc = S.tet_local[tb + lt];
-    if (!first) lam = S.lam[tb + lt];
-  }
-  int2 vsum = make_int2(0, 0);
-  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
-    const int g = S.p_vert_ids[vb + i];
-    vsum = S.p_vert_sum[vb + i];
-    s_y[i] = S.pos[g];
-    s_rest[i] = S.rest[g];
-    s_base[i] = vsum.x & 0xffff;
-    if (!csr) s_acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-  __syncthreads();
-  if (lt < nt) {
-    const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu), int((loc.x >> 16) & 0xffu),
-                      int(loc.x >> 24)};
-    const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
-    Mat3 idm;
-    float V0;
-    rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
-    TetProjection P;
-    project_tet(S, q, idm, V0, lam, P);
-    S.lam[tb + lt] = lam;
-    if (csr) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int rank = int((loc.y >> (8 * k)) & 0xffu);
-        s_sorted[s_base[l[k]] + rank] = make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        atomicAdd(&s_acc[l[k]].x, P.dp[k].x);
-        atomicAdd(&s_acc[l[k]].y, P.dp[k].y);
-        atomicAdd(&s_acc[l[k]].z, P.dp[k].z);
-        if (!S.block_mode) atomicAdd(&s_acc[l[k]].w, P.cnt[k]);
-      }
+  const int tid = threadIdx.x;
+
+  // stage 0 (current): everything in registers
+  PatchItem h0 = items[rg.x];
+  int2 vs0 = make_int2(0, 0);
+  float4 y0 = make_float4(0.f, 0.f, 0.f, 0.f), r0 = y0;
+  uint2 loc0 = make_uint2(0u, 0u);
+  float2 lam0 = make_float2(0.f, 0.f);
+  {
+    const SceneDev& S = scenes[h0.scene];
+    if (tid < h0.n_verts) {
+      const int g = S.p_vert_ids[h0.vert_begin + tid];
+      vs0 = S.p_vert_sum[h0.vert_begin + tid];
+      y0 = stage_iterate(S, g, it, vs0.x < 0);
+      r0 = S.rest[g];
+    }
+    if (tid < h0.n_tets) {
+      loc0 = S.tet_local[h0.tet_begin + tid];
+      if (!first) lam0 = S.lam[h0.tet_begin + tid];
     }
   }
-  __syncthreads();
-  // nv <= blockDim.x: thread i still holds vertex i's record in vsum
-  if (threadIdx.x < nv) {
-    const int i = threadIdx.x;
-    float4 a;
-    if (csr) {
-      a = make_float4(0.f, 0.f, 0.f, 0.f);
-      const int cb = vsum.x & 0xffff, ce = cb + (vsum.x >> 16);
-      for (int c = cb; c < ce; ++c) {
-        const float4 v = s_sorted[c];
-        a.x += v.x;
-        a.y += v.y;
-        a.z += v.z;
-        a.w += v.w;
-      }
-    } else {
-      a = s_acc[i];
+  // stage 1 (next): vertex ids only
+  bool has1 = rg.x + 1 < rg.y;
+  PatchItem h1 = h0;
+  int g1 = 0;
+  int2 vs1 = make_int2(0, 0);
+  if (has1) {
+    h1 = items[rg.x + 1];
+    const SceneDev& S = scenes[h1.scene];
+    if (tid < h1.n_verts) {
+      g1 = S.p_vert_ids[h1.vert_begin + tid];
+      vs1 = S.p_vert_sum[h1.vert_begin + tid];
     }
-    S.slot_acc[vsum.y] = a;
+  }
+  for (int i = rg.x; i < rg.y; ++i) {
+    const SceneDev& S = scenes[h0.scene];
+    const bool csr = S.csr != 0;
+    if (tid < h0.n_verts) {
+      s_y[tid] = y0;
+      s_rest[tid] = r0;
+      s_base[tid] = vs0.x & 0xffff;
+      if (!csr) s_acc[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    // in flight during the compute below: data of patch i+1, ids of patch i+2
+    float4 y1 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = y1;
+    uint2 loc1 = make_uint2(0u, 0u);
+    float2 lam1 = make_float2(0.f, 0.f);
+    if (has1) {
+      const SceneDev& S1 = scenes[h1.scene];
+      if (tid < h1.n_verts) {
+        y1 = stage_iterate(S1, g1, it, vs1.x < 0);
+        r1 = S1.rest[g1];
+      }
+      if (tid < h1.n_tets) {
+        loc1 = S1.tet_local[h1.tet_begin + tid];
+        if (!first) lam1 = S1.lam[h1.tet_begin + tid];
+      }
+    }
+    const bool has2 = i + 2 < rg.y;
+    PatchItem h2 = h1;
+    int g2 = 0;
+    int2 vs2 = make_int2(0, 0);
+    if (has2) {
+      h2 = items[i + 2];
+      const SceneDev& S2 = scenes[h2.scene];
+      if (tid < h2.n_verts) {
+        g2 = S2.p_vert_ids[h2.vert_begin + tid];
+        vs2 = S2.p_vert_sum[h2.vert_begin + tid];
+      }
+    }
+    if (tid < h0.n_tets) {
+      const int l[4] = {int(loc0.x & 0xffu), int((loc0.x >> 8) & 0xffu),
+                        int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
+      const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
+      Mat3 idm;
+      float V0;
+      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
+      float2 lam = lam0;
+      TetProjection P;
+      project_tet(S, q, idm, V0, lam, P);
+      S.lam[h0.tet_begin + tid] = lam;
+      if (csr) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int rank = int((loc0.y >> (8 * k)) & 0xffu);
+          s_sorted[s_base[l[k]] + rank] = make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          atomicAdd(&s_acc[l[k]].x, P.dp[k].x);
+          atomicAdd(&s_acc[l[k]].y, P.dp[k].y);
+          atomicAdd(&s_acc[l[k]].z, P.dp[k].z);
+          if (!S.block_mode) atomicAdd(&s_acc[l[k]].w, P.cnt[k]);
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < h0.n_verts) {
+      float4 a;
+      if (csr) {
+        a = make_float4(0.f, 0.f, 0.f, 0.f);
+        const int cb = vs0.x & 0xffff, ce = cb + ((vs0.x >> 16) & 0x7fff);
+        for (int c = cb; c < ce; ++c) {
+          const float4 v = s_sorted[c];
+          a.x += v.x;
+          a.y += v.y;
+          a.z += v.z;
+          a.w += v.w;
+        }
+      } else {
+        a = s_acc[tid];
+      }
+      S.slot_acc[vs0.y] = a;
+    }
+    __syncthreads();  // the LDS arrays are refilled by the next patch
+    h0 = h1;
+    vs0 = vs1;
+    y0 = y1;
+    r0 = r1;
+    loc0 = loc1;
+    lam0 = lam1;
+    h1 = h2;
+    g1 = g2;
+    vs1 = vs2;
+    has1 = has2;
   }
 }
 
 // Jacobi average from the slots (cheb: into xj; otherwise pos in place).
 __global__ void k_patch_average(const SceneDev* __restrict__ scenes,
-                                const WorkItem* __restrict__ map) {
+                                const WorkItem* __restrict__ map, int it) {
   const WorkItem wi = map[blockIdx.x];
   const SceneDev& S = scenes[wi.scene];
   const int v = wi.first + threadIdx.x;
   if (v >= S.n_verts) return;
-  const float4 out = jacobi_average_slots(S, v);
+  const float4 out = jacobi_average_slots(S, v, it);
   if (S.cheb)
     S.xj[v] = out;
   else
     S.pos[v] = out;
 }
 
-// Residual per patch: stage x_J and rest positions of the patch in LDS.
+// x_J of vertex g as staged by the residual: the Jacobi average in xj
+// (Chebyshev on) or pos (off), written by k_patch_average.
+__device__ __forceinline__ float4 stage_jacobi(const SceneDev& S, int g) {
+  return S.cheb ? S.xj[g] : S.pos[g];
+}
+
+// Persistent patch residual, pipelined like k_patch_project: x_J (Chebyshev
+// on) or pos (off) and the rest positions of a patch are staged in LDS, one
+// partial (r, |C_H|, |C_D|) per patch is written to the scene's partials.
 __global__ void __launch_bounds__(kPatchThreads)
-    k_patch_residual(const SceneDev* __restrict__ scenes, const WorkItem* __restrict__ map) {
-  const WorkItem wi = map[blockIdx.x];
-  const SceneDev& S = scenes[wi.scene];
-  const int p = wi.first;
-  const int tb = S.p_tet_begin[p], nt = S.p_tet_begin[p + 1] - tb;
-  const int vb = S.p_vert_begin[p], nv = S.p_vert_begin[p + 1] - vb;
+    k_patch_residual(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
+                     const int2* __restrict__ ranges) {
+  const int2 rg = ranges[blockIdx.x];
+  if (rg.x >= rg.y) return;
   __shared__ float4 s_x[kPatchVerts];
   __shared__ float4 s_rest[kPatchVerts];
-  const float4* X = S.cheb ? S.xj : S.pos;
-  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
-    const int g = S.p_vert_ids[vb + i];
-    s_x[i] = X[g];
-    s_rest[i] = S.rest[g];
+  const int tid = threadIdx.x;
+  PatchItem h0 = items[rg.x];
+  float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), r0 = x0;
+  uint2 loc0 = make_uint2(0u, 0u);
+  float2 lam0 = make_float2(0.f, 0.f);
+  {
+    const SceneDev& S = scenes[h0.scene];
+    if (tid < h0.n_verts) {
+      const int g = S.p_vert_ids[h0.vert_begin + tid];
+      x0 = stage_jacobi(S, g);
+      r0 = S.rest[g];
+    }
+    if (tid < h0.n_tets) {
+      loc0 = S.tet_local[h0.tet_begin + tid];
+      lam0 = S.lam[h0.tet_begin + tid];
+    }
   }
-  __syncthreads();
-  float r = 0.f, ah = 0.f, ad = 0.f;
-  const int lt = threadIdx.x;
-  if (lt < nt) {
-    const int t = tb + lt;
-    const unsigned loc = S.tet_local[t].x;
-    const int l[4] = {int(loc & 0xffu), int((loc >> 8) & 0xffu), int((loc >> 16) & 0xffu),
-                      int(loc >> 24)};
-    Mat3 idm;
-    float V0;
-    rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
-    residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, V0, S.lam[t],
-                 r, ah, ad);
+  bool has1 = rg.x + 1 < rg.y;
+  PatchItem h1 = h0;
+  int g1 = 0;
+  if (has1) {
+    h1 = items[rg.x + 1];
+    if (tid < h1.n_verts) g1 = scenes[h1.scene].p_vert_ids[h1.vert_begin + tid];
   }
-  store_partial(S, p, r, ah, ad);
+  for (int i = rg.x; i < rg.y; ++i) {
+    const SceneDev& S = scenes[h0.scene];
+    if (tid < h0.n_verts) {
+      s_x[tid] = x0;
+      s_rest[tid] = r0;
+    }
+    __syncthreads();
+    float4 x1 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = x1;
+    uint2 loc1 = make_uint2(0u, 0u);
+    float2 lam1 = make_float2(0.f, 0.f);
+    if (has1) {
+      const SceneDev& S1 = scenes[h1.scene];
+      if (tid < h1.n_verts) {
+        x1 = stage_jacobi(S1, g1);
+        r1 = S1.rest[g1];
+      }
+      if (tid < h1.n_tets) {
+        loc1 = S1.tet_local[h1.tet_begin + tid];
+        lam1 = S1.lam[h1.tet_begin + tid];
+      }
+    }
+    const bool has2 = i + 2 < rg.y;
+    PatchItem h2 = h1;
+    int g2 = 0;
+    if (has2) {
+      h2 = items[i + 2];
+      if (tid < h2.n_verts) g2 = scenes[h2.scene].p_vert_ids[h2.vert_begin + tid];
+    }
+    float r = 0.f, ah = 0.f, ad = 0.f;
+    if (tid < h0.n_tets) {
+      const int l[4] = {int(loc0.x & 0xffu), int((loc0.x >> 8) & 0xffu),
+                        int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
+      Mat3 idm;
+      float V0;
+      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
+      residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, V0, lam0,
+                   r, ah, ad);
+    }
+    store_partial(S, h0.patch, r, ah, ad);  // ends with a workgroup barrier
+    __syncthreads();
+    h0 = h1;
+    x0 = x1;
+    r0 = r1;
+    loc0 = loc1;
+    lam0 = lam1;
+    h1 = h2;
+    g1 = g2;
+    has1 = has2;
+  }
 }
 
 // ---- per-scene scalar control ----------------------------------------------------
@@ -567,13 +720,26 @@ __global__ void k_patch_blend(const SceneDev* __restrict__ scenes,
   if (v >= S.n_verts) return;
   const float omega = float(S.ctrl->omega);
   const float gamma = S.gamma;
-  const float4 xj4 = jacobi_average_slots(S, v);
+  const float4 xj4 = jacobi_average_slots(S, v, 0);
   const float4 y4 = S.pos[v];
   const float3 y = f3(y4), xj = f3(xj4), pr = f3(S.prev[v]);
   const float3 x_new = gamma * (xj - y) + y;
   const float3 nw = pr + omega * (x_new - pr);
   S.prev[v] = make_float4(y.x, y.y, y.z, 0.f);
   S.pos[v] = make_float4(nw.x, nw.y, nw.z, y4.w);
+}
+
+// Blend of the last iteration of the fused sequence: y^n_iter into pos.
+__global__ void k_final_blend(const SceneDev* __restrict__ scenes,
+                              const WorkItem* __restrict__ map, int n_iter) {
+  const WorkItem wi = map[blockIdx.x];
+  const SceneDev& S = scenes[wi.scene];
+  const int v = wi.first + threadIdx.x;
+  if (v >= S.n_verts || n_iter <= 0) return;
+  const int k = n_iter - 1;
+  const float4 y = S.ybuf[k % 3][v];
+  const float4 pr = k == 0 ? y : S.ybuf[(k - 1) % 3][v];
+  S.pos[v] = cheb_blend(S, y, S.xj[v], pr, float(S.ctrl->omega));
 }
 
 // advance (xpbd_base.py:148-160).
@@ -646,20 +812,29 @@ hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nbl
   k_residual<<<nblocks, bs, 0, st>>>(scenes, tmap);
   return hipGetLastError();
 }
-hipError_t launch_patch_project(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
-                                int first, hipStream_t st) {
-  k_patch_project<<<nblocks, kPatchThreads, 0, st>>>(scenes, pmap, first);
+hipError_t launch_patch_project(const SceneDev* scenes, const PatchItem* items,
+                                const int2* ranges, int nblocks, int it, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  k_patch_project<<<nblocks, kPatchThreads, 0, st>>>(scenes, items, ranges, it);
   return hipGetLastError();
 }
 hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks,
-                                int bs, hipStream_t st) {
-  k_patch_average<<<nblocks, bs, 0, st>>>(scenes, vmap);
+                                int bs, int it, hipStream_t st) {
+  k_patch_average<<<nblocks, bs, 0, st>>>(scenes, vmap, it);
   return hipGetLastError();
 }
-hipError_t launch_patch_residual(const SceneDev* scenes, const WorkItem* pmap, int nblocks,
-                                 hipStream_t st) {
-  k_patch_residual<<<nblocks, kPatchThreads, 0, st>>>(scenes, pmap);
+hipError_t launch_patch_residual(const SceneDev* scenes, const PatchItem* items,
+                                 const int2* ranges, int nblocks, hipStream_t st) {
+  if (nblocks <= 0) return hipSuccess;
+  k_patch_residual<<<nblocks, kPatchThreads, 0, st>>>(scenes, items, ranges);
   return hipGetLastError();
+}
+hipError_t patch_kernel_occupancy(int* project_blocks_per_cu, int* residual_blocks_per_cu) {
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(project_blocks_per_cu,
+                                                              k_patch_project, kPatchThreads, 0);
+  if (e != hipSuccess) return e;
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(residual_blocks_per_cu, k_patch_residual,
+                                                      kPatchThreads, 0);
 }
 hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st) {
   k_reduce<<<n_scenes, 1024, 0, st>>>(scenes, it);
@@ -673,6 +848,11 @@ hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblock
 hipError_t launch_patch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                               hipStream_t st) {
   k_patch_blend<<<nblocks, bs, 0, st>>>(scenes, vmap);
+  return hipGetLastError();
+}
+hipError_t launch_final_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
+                              int n_iter, hipStream_t st) {
+  k_final_blend<<<nblocks, bs, 0, st>>>(scenes, vmap, n_iter);
   return hipGetLastError();
 }
 hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,

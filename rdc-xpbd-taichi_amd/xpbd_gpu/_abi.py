@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 LIB_NAME = "libxpbd_hip.so"
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+# XPBD_LIB selects another build of the library (for example a debug build).
+LIB_PATH = os.environ.get("XPBD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                      LIB_NAME)
 
 XPBD_OK = 0
 XPBD_EINVAL = -1
@@ -168,7 +170,7 @@ def load() -> ctypes.CDLL:
         "xpbd_get_kernel_time": [vp, dp, ctypes.POINTER(ctypes.c_int64)],
         "xpbd_check_finite": [vp],
         "xpbd_last_error": [ctypes.c_char_p, ctypes.c_size_t],
-        "xpbd_patch_stats": [ctypes.POINTER(ctypes.c_int32), i32, i32,
+        "xpbd_patch_stats": [fp, ctypes.POINTER(ctypes.c_int32), i32, i32,
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],
     }
@@ -211,15 +213,18 @@ def dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def patch_stats(tets: np.ndarray, n_verts: int) -> dict:
-    """Patch layout the library would build for these tets (host only)."""
+def patch_stats(tets: np.ndarray, n_verts: int, positions=None) -> dict:
+    """Patch layout the library would build for these tets (host only).  With
+    positions, the solver's bisection ordering is applied first."""
     lib = load()
     t = np.ascontiguousarray(tets, dtype=np.int32).reshape(-1, 4)
+    pos = None if positions is None else np.ascontiguousarray(positions, dtype=np.float32)
     npch = ctypes.c_int32()
     nsl = ctypes.c_int64()
     mv = ctypes.c_int32()
     mt = ctypes.c_int32()
-    check(lib.xpbd_patch_stats(t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), t.shape[0],
+    check(lib.xpbd_patch_stats(None if pos is None else fptr(pos),
+                               t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), t.shape[0],
                                int(n_verts), ctypes.byref(npch), ctypes.byref(nsl),
                                ctypes.byref(mv), ctypes.byref(mt)))
     return {"n_patches": npch.value, "n_slots": nsl.value, "max_patch_verts": mv.value,

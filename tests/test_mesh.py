@@ -77,9 +77,22 @@ def test_patch_builder_invariants():
     if not _abi.library_available():
         pytest.skip("libxpbd_hip.so not built")
     for m in (box_grid(1), box_grid(5, 3, 7), box_grid(16, jitter=0.2, seed=2)):
-        st = _abi.patch_stats(m.tets, m.n_verts)
-        assert st["max_patch_tets"] <= 256 and st["max_patch_verts"] <= 256
-        assert st["n_patches"] >= (m.n_tets + 255) // 256
-        assert st["n_slots"] >= m.n_verts
+        for pos in (None, m.verts):
+            st = _abi.patch_stats(m.tets, m.n_verts, pos)
+            assert st["max_patch_tets"] <= 256 and st["max_patch_verts"] <= 256
+            assert st["n_patches"] >= (m.n_tets + 255) // 256
+            assert st["n_slots"] >= m.n_verts
     with pytest.raises(ValueError):
         _abi.patch_stats(np.array([[0, 1, 2, 9]]), 4)
+
+
+def test_bisection_patches_are_compact():
+    """Coordinate bisection stages far fewer vertices per tet than greedy cuts
+    of the given order."""
+    from xpbd_gpu import _abi
+    if not _abi.library_available():
+        pytest.skip("libxpbd_hip.so not built")
+    m = box_grid(24, jitter=0.2, seed=0)
+    greedy = _abi.patch_stats(m.tets, m.n_verts)
+    rcb = _abi.patch_stats(m.tets, m.n_verts, m.verts)
+    assert rcb["n_slots"] < 0.8 * greedy["n_slots"]
