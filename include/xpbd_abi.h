@@ -192,6 +192,43 @@ int xpbd_get_kernel_time(xpbd_solver* s, double* total_ms, int64_t* launches);
  * infinite; the reference never checks (xpbd_base.py:139-160). */
 int xpbd_check_finite(xpbd_solver* s);
 
+/* Device memory held by one solver handle, in bytes (for admission by an
+ * HBM budget; no reference counterpart). */
+int xpbd_get_device_bytes(xpbd_solver* s, int64_t* bytes);
+
+/* ---- Batched form (no reference counterpart) -------------------------------
+ * Many independent scenes advanced together: every kernel launch of a batch
+ * substep covers all member scenes (their patches and vertex blocks are
+ * concatenated into one work list), and the substep (or a whole frame) is
+ * replayed from one hipGraph.  Members must use the patch layout and share
+ * constraints_iter, substeps_num, frame_dt, dt and the mode flags; material,
+ * mesh, SDFs and masses are per scene.  Results of a scene are bit-identical
+ * alone and in any batch (deterministic accumulation).  While a batch runs,
+ * member scenes must not be stepped on their own streams;
+ * xpbd_batch_synchronize() before reading their state. */
+typedef struct xpbd_batch xpbd_batch;
+
+typedef struct xpbd_batch_opts {
+  int32_t graph;          /* 1: replay captured graphs (default), 0: launch kernels */
+  int32_t graph_substeps; /* substeps per graph: 1 (default) or substeps_num (a frame) */
+  int32_t vertex_block;   /* threads per vertex workgroup, 64..1024, multiple of 64 */
+  int32_t grid_pct;       /* persistent patch grid, percent of full occupancy, 1..100 */
+  int32_t xcd_split;      /* 1: every XCD walks a contiguous share of the work (default) */
+} xpbd_batch_opts;
+
+int xpbd_batch_opts_init(xpbd_batch_opts* o);
+int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_batch_opts* opts,
+                      xpbd_batch** out);
+int xpbd_batch_destroy(xpbd_batch* b);
+/* one substep (sub_dt = dt / substeps_num of each scene) of every member */
+int xpbd_batch_substep(xpbd_batch* b);
+/* n_frames frames of every member (solve(), neohookean_solver.py:562-587) */
+int xpbd_batch_frame(xpbd_batch* b, int32_t n_frames);
+int xpbd_batch_synchronize(xpbd_batch* b);
+int xpbd_batch_get_stream(xpbd_batch* b, void** stream);
+int xpbd_batch_info(xpbd_batch* b, int32_t* n_scenes, int64_t* n_verts, int64_t* n_tets,
+                    int32_t* n_patches);
+
 /* Host-only description of the patch layout a mesh would get (no device
  * work): number of patches, number of (patch, vertex) slots, and the largest
  * vertex and tet counts of a patch.  With positions (n_verts*3) the tets are

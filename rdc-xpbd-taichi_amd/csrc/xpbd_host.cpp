@@ -342,6 +342,7 @@ struct xpbd_solver {
   DevBuf<int2> p_vert_sum;
   DevBuf<uint2> tet_local;
   DevBuf<xpbd::PatchItem> pitems;
+  std::vector<PatchItem> pitems_host;
   DevBuf<int2> prange_proj, prange_res;
   int grid_proj = 0, grid_res = 0;
   int n_patches = 0;
@@ -376,6 +377,18 @@ struct xpbd_solver {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
   double timed_ms = 0.0;
   int64_t timed_launches = 0;
+
+  int64_t device_bytes() const {
+    int64_t b = 0;
+    auto add = [&](const auto& buf) { b += int64_t(buf.n) * int64_t(sizeof(*buf.p)); };
+    add(pos); add(xs); add(vel); add(prev); add(xj); add(dacc); add(corner); add(rest);
+    add(slot_acc); add(ybuf1); add(ybuf2); add(p_tet_begin); add(p_vert_begin); add(p_vert_ids);
+    add(vslot_off); add(p_vert_sum); add(tet_local); add(pitems); add(prange_proj);
+    add(prange_res); add(valence); add(adj_off); add(adj); add(d_old2new); add(tet); add(dm0);
+    add(dm1); add(dm2); add(lam); add(partial); add(logbuf); add(ctrl); add(table); add(vmap);
+    add(tmap); add(scratch_xyz); add(flag);
+    return b;
+  }
 
   ~xpbd_solver() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -984,6 +997,7 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
       it.n_verts = P.vert_begin[p + 1] - P.vert_begin[p];
     }
     TRY(s->pitems.upload(items.data(), items.size()));
+    s->pitems_host = items;
     int occ_p = 0, occ_r = 0, n_cu = 0;
     HIP_TRY(xpbd::patch_kernel_occupancy(&occ_p, &occ_r));
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, d->device));
@@ -1439,6 +1453,278 @@ int xpbd_check_finite(xpbd_solver* s) {
   HIP_TRY(hipMemcpyAsync(&hit, s->flag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (hit) return set_error(XPBD_ENUMERIC, "non-finite vertex position in the solver state");
+  return XPBD_OK;
+}
+
+}  // extern "C"
+
+// ---- batched form ------------------------------------------------------------
+
+struct xpbd_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<xpbd_solver*> scenes;
+  xpbd_batch_opts opts{};
+  DevBuf<SceneDev> table;
+  std::vector<SceneDev> table_host;
+  DevBuf<WorkItem> vmap;
+  DevBuf<PatchItem> pitems;
+  DevBuf<int2> prange_proj, prange_res;
+  int nvb = 0, n_items = 0, grid_proj = 0, grid_res = 0;
+  int64_t n_verts = 0, n_tets = 0;
+  SceneDev proto{};  // flags and iteration count shared by the members
+  hipGraphExec_t gexec = nullptr;
+  int graph_substeps = 0;
+  int pos_in_frame = 0;  // substeps of the current frame already run
+
+  ~xpbd_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int batch_sync_table(xpbd_batch* b) {
+  bool changed = b->table_host.size() != b->scenes.size();
+  b->table_host.resize(b->scenes.size());
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    xpbd_solver* s = b->scenes[i];
+    SceneDev S = s->host_scene;  // the member's buffers, with its current parameters
+    fill_scene_params(s, S, s->prm.dt / s->prm.substeps_num, s->prm.damping);
+    if (changed || std::memcmp(&S, &b->table_host[i], sizeof(S)) != 0) {
+      b->table_host[i] = S;
+      changed = true;
+    }
+  }
+  if (!changed) return XPBD_OK;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(b->table.p, b->table_host.data(), sizeof(SceneDev) * b->table_host.size(),
+                    hipMemcpyHostToDevice));
+  return XPBD_OK;
+}
+
+// One substep of every member: predict + collide + save, the Jacobi
+// iterations, the final blend and advance (the fused patch sequence of
+// enqueue_iterations, over the concatenated work list).
+int batch_enqueue_substep(xpbd_batch* b) {
+  const SceneDev& P = b->proto;
+  const SceneDev* T = b->table.p;
+  const int bs = b->opts.vertex_block;
+  const int ns = int(b->scenes.size());
+  HIP_TRY(xpbd::launch_predict(T, b->vmap.p, b->nvb, bs, 1, 1, P.cheb, b->stream));
+  for (int it = 0; it < P.iters; ++it) {
+    HIP_TRY(xpbd::launch_patch_project(T, b->pitems.p, b->prange_proj.p, b->grid_proj, it,
+                                       b->stream));
+    HIP_TRY(xpbd::launch_patch_average(T, b->vmap.p, b->nvb, bs, it, b->stream));
+    if (P.cheb || P.monitor) {
+      HIP_TRY(xpbd::launch_patch_residual(T, b->pitems.p, b->prange_res.p, b->grid_res,
+                                          b->stream));
+      HIP_TRY(xpbd::launch_reduce(T, ns, it, b->stream));
+    }
+    if (P.cheb && !P.fused) HIP_TRY(xpbd::launch_blend(T, b->vmap.p, b->nvb, bs, b->stream));
+  }
+  if (P.fused && P.iters > 0)
+    HIP_TRY(xpbd::launch_final_blend(T, b->vmap.p, b->nvb, bs, P.iters, b->stream));
+  HIP_TRY(xpbd::launch_advance(T, b->vmap.p, b->nvb, bs, b->stream));
+  return XPBD_OK;
+}
+
+void batch_count_substep(xpbd_batch* b) {
+  for (xpbd_solver* s : b->scenes) {
+    s->substep = b->pos_in_frame;
+  }
+  b->pos_in_frame += 1;
+  const xpbd_desc& d = b->scenes[0]->prm;
+  if (b->pos_in_frame == d.substeps_num) {
+    b->pos_in_frame = 0;
+    for (xpbd_solver* s : b->scenes) {
+      s->time += s->prm.dt;
+      s->frame += 1;
+    }
+  }
+}
+
+int batch_run(xpbd_batch* b, int n_substeps) {
+  TRY(batch_sync_table(b));
+  if (!b->opts.graph) {
+    for (int k = 0; k < n_substeps; ++k) {
+      TRY(batch_enqueue_substep(b));
+      batch_count_substep(b);
+    }
+    return XPBD_OK;
+  }
+  const int per = b->opts.graph_substeps;
+  int done = 0;
+  while (done < n_substeps) {
+    const int chunk = (per > 1 && n_substeps - done >= per && b->pos_in_frame == 0) ? per : 1;
+    if (!b->gexec || b->graph_substeps != chunk) {
+      if (b->gexec) {
+        HIP_TRY(hipGraphExecDestroy(b->gexec));
+        b->gexec = nullptr;
+      }
+      hipGraph_t g;
+      HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = XPBD_OK;
+      for (int k = 0; k < chunk && rc == XPBD_OK; ++k) rc = batch_enqueue_substep(b);
+      const hipError_t ec = hipStreamEndCapture(b->stream, &g);
+      if (rc != XPBD_OK) return rc;
+      HIP_TRY(ec);
+      const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ei);
+      b->graph_substeps = chunk;
+    }
+    HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+    for (int k = 0; k < chunk; ++k) batch_count_substep(b);
+    done += chunk;
+  }
+  return XPBD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xpbd_get_device_bytes(xpbd_solver* s, int64_t* bytes) {
+  if (!s || !bytes) return set_error(XPBD_EINVAL, "NULL argument");
+  *bytes = s->device_bytes();
+  return XPBD_OK;
+}
+
+int xpbd_batch_opts_init(xpbd_batch_opts* o) {
+  if (!o) return set_error(XPBD_EINVAL, "opts is NULL");
+  o->graph = 1;
+  o->graph_substeps = 1;
+  o->vertex_block = 256;
+  o->grid_pct = 100;
+  o->xcd_split = 1;
+  return XPBD_OK;
+}
+
+int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_batch_opts* opts,
+                      xpbd_batch** out) {
+  if (!scenes || n_scenes <= 0 || !out) return set_error(XPBD_EINVAL, "batch needs scenes");
+  *out = nullptr;
+  auto b = std::make_unique<xpbd_batch>();
+  if (opts)
+    b->opts = *opts;
+  else
+    xpbd_batch_opts_init(&b->opts);
+  const xpbd_batch_opts& o = b->opts;
+  if (o.vertex_block < 64 || o.vertex_block > 1024 || o.vertex_block % 64)
+    return set_error(XPBD_EINVAL, "vertex_block must be a multiple of 64 in [64, 1024]");
+  if (o.grid_pct < 1 || o.grid_pct > 100) return set_error(XPBD_EINVAL, "grid_pct in [1, 100]");
+  xpbd_solver* s0 = scenes[0];
+  if (!s0) return set_error(XPBD_EINVAL, "NULL scene");
+  b->device = s0->device;
+  HIP_TRY(hipSetDevice(b->device));
+  for (int i = 0; i < n_scenes; ++i) {
+    xpbd_solver* s = scenes[i];
+    if (!s) return set_error(XPBD_EINVAL, "NULL scene %d", i);
+    for (int j = 0; j < i; ++j)
+      if (scenes[j] == s) return set_error(XPBD_EINVAL, "scene %d appears twice", i);
+    if (s->device != b->device) return set_error(XPBD_EINVAL, "scene %d on another device", i);
+    if (s->layout != XPBD_LAYOUT_PATCH) return set_error(XPBD_EINVAL, "scene %d: batches need the patch layout", i);
+    const xpbd_desc &a = s->prm, &r = s0->prm;
+    if (a.constraints_iter != r.constraints_iter || a.substeps_num != r.substeps_num ||
+        a.frame_dt != r.frame_dt || a.dt != r.dt || a.cheb_enable != r.cheb_enable ||
+        a.block_neohookean_enable != r.block_neohookean_enable ||
+        a.monitor_convergence != r.monitor_convergence || a.accumulate_mode != r.accumulate_mode ||
+        s->has_isolated != s0->has_isolated)
+      return set_error(XPBD_EINVAL, "scene %d: iteration count, time step or mode flags differ", i);
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    b->scenes.push_back(s);
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  // concatenated work lists, in member order
+  std::vector<WorkItem> vm;
+  std::vector<PatchItem> items;
+  for (int i = 0; i < n_scenes; ++i) {
+    xpbd_solver* s = b->scenes[i];
+    for (int v = 0; v < s->nv; v += o.vertex_block) vm.push_back({i, v});
+    for (PatchItem it : s->pitems_host) {
+      it.scene = i;
+      items.push_back(it);
+    }
+    b->n_verts += s->nv;
+    b->n_tets += s->nt;
+  }
+  const int nx = o.xcd_split ? xcd_count() : 1;
+  std::vector<WorkItem> vmo(vm.size());
+  const std::vector<int> order = xcd_order(int(vm.size()), nx);
+  for (size_t k = 0; k < vm.size(); ++k) vmo[k] = vm[order[k]];
+  b->nvb = int(vmo.size());
+  b->n_items = int(items.size());
+  TRY(b->vmap.upload(vmo.data(), vmo.size()));
+  TRY(b->pitems.upload(items.data(), items.size()));
+  int occ_p = 0, occ_r = 0, n_cu = 0;
+  HIP_TRY(xpbd::patch_kernel_occupancy(&occ_p, &occ_r));
+  HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
+  const long long full_p = (long long)n_cu * std::max(1, occ_p);
+  const long long full_r = (long long)n_cu * std::max(1, occ_r);
+  b->grid_proj = int(std::max(1LL, std::min<long long>(b->n_items, full_p * o.grid_pct / 100)));
+  b->grid_res = int(std::max(1LL, std::min<long long>(b->n_items, full_r * o.grid_pct / 100)));
+  std::vector<int2> rp, rr;
+  TRY(xcd_ranges(b->n_items, b->grid_proj, nx, rp));
+  TRY(xcd_ranges(b->n_items, b->grid_res, nx, rr));
+  TRY(b->prange_proj.upload(rp.data(), rp.size()));
+  TRY(b->prange_res.upload(rr.data(), rr.size()));
+  TRY(b->table.alloc(n_scenes));
+  TRY(batch_sync_table(b.get()));
+  b->proto = b->table_host[0];
+  b->pos_in_frame = 0;
+  *out = b.release();
+  return XPBD_OK;
+}
+
+int xpbd_batch_destroy(xpbd_batch* b) {
+  if (!b) return XPBD_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  delete b;
+  return XPBD_OK;
+}
+
+int xpbd_batch_substep(xpbd_batch* b) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  HIP_TRY(hipSetDevice(b->device));
+  return batch_run(b, 1);
+}
+
+int xpbd_batch_frame(xpbd_batch* b, int32_t n_frames) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  HIP_TRY(hipSetDevice(b->device));
+  const xpbd_desc& d = b->scenes[0]->prm;
+  // frame chunking of solve() (neohookean_solver.py:566-571); with dt ==
+  // frame_dt a frame is exactly substeps_num substeps
+  if (d.dt != d.frame_dt)
+    return set_error(XPBD_EINVAL, "batch frames need dt == frame_dt; step substeps instead");
+  if (b->pos_in_frame != 0)
+    return set_error(XPBD_ESTATE, "a frame is partly stepped; finish it with xpbd_batch_substep");
+  return batch_run(b, n_frames * d.substeps_num);
+}
+
+int xpbd_batch_synchronize(xpbd_batch* b) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  HIP_TRY(hipSetDevice(b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return XPBD_OK;
+}
+
+int xpbd_batch_get_stream(xpbd_batch* b, void** stream) {
+  if (!b || !stream) return set_error(XPBD_EINVAL, "NULL argument");
+  *stream = static_cast<void*>(b->stream);
+  return XPBD_OK;
+}
+
+int xpbd_batch_info(xpbd_batch* b, int32_t* n_scenes, int64_t* n_verts, int64_t* n_tets,
+                    int32_t* n_patches) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  if (n_scenes) *n_scenes = int32_t(b->scenes.size());
+  if (n_verts) *n_verts = b->n_verts;
+  if (n_tets) *n_tets = b->n_tets;
+  if (n_patches) *n_patches = b->n_items;
   return XPBD_OK;
 }
 

@@ -45,7 +45,9 @@ EXPORTED = (
     "xpbd_get_stream", "xpbd_set_logging", "xpbd_log_size", "xpbd_get_log", "xpbd_clear_log",
     "xpbd_get_state", "xpbd_set_state", "xpbd_get_sizes", "xpbd_set_exec_path",
     "xpbd_set_kernel_timing", "xpbd_get_kernel_time", "xpbd_check_finite", "xpbd_last_error",
-    "xpbd_patch_stats",
+    "xpbd_patch_stats", "xpbd_get_device_bytes", "xpbd_batch_opts_init", "xpbd_batch_create",
+    "xpbd_batch_destroy", "xpbd_batch_substep", "xpbd_batch_frame", "xpbd_batch_synchronize",
+    "xpbd_batch_get_stream", "xpbd_batch_info",
 )
 
 
@@ -89,6 +91,16 @@ class XpbdDesc(ctypes.Structure):
         ("block_size", ctypes.c_int32),
         ("device", ctypes.c_int32),
         ("layout", ctypes.c_int32),
+    ]
+
+
+class XpbdBatchOpts(ctypes.Structure):
+    _fields_ = [
+        ("graph", ctypes.c_int32),
+        ("graph_substeps", ctypes.c_int32),
+        ("vertex_block", ctypes.c_int32),
+        ("grid_pct", ctypes.c_int32),
+        ("xcd_split", ctypes.c_int32),
     ]
 
 
@@ -170,6 +182,17 @@ def load() -> ctypes.CDLL:
         "xpbd_get_kernel_time": [vp, dp, ctypes.POINTER(ctypes.c_int64)],
         "xpbd_check_finite": [vp],
         "xpbd_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+        "xpbd_get_device_bytes": [vp, ctypes.POINTER(ctypes.c_int64)],
+        "xpbd_batch_opts_init": [ctypes.POINTER(XpbdBatchOpts)],
+        "xpbd_batch_create": [ctypes.POINTER(ctypes.c_void_p), i32, ctypes.POINTER(XpbdBatchOpts),
+                              pp],
+        "xpbd_batch_destroy": [vp],
+        "xpbd_batch_substep": [vp],
+        "xpbd_batch_frame": [vp, i32],
+        "xpbd_batch_synchronize": [vp],
+        "xpbd_batch_get_stream": [vp, pp],
+        "xpbd_batch_info": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
+                            ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)],
         "xpbd_patch_stats": [fp, ctypes.POINTER(ctypes.c_int32), i32, i32,
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],

@@ -1,0 +1,168 @@
+"""Batched form: many independent scenes advanced by shared kernel launches.
+
+A `SceneBatch` concatenates the patches and vertex blocks of its member
+`XPBDNeoHookeanSolver` scenes into one work list, so each kernel of a substep
+(and one replayed hipGraph per substep or per frame) covers every scene.
+Scenes keep their own buffers, materials, SDFs and rho/omega state; the
+results of a scene are bit-identical alone and in any batch.
+
+`autotune` picks the batch's launch knobs (vertex workgroup size, persistent
+grid size, XCD-contiguous work split, substeps per graph) by timing them on the
+actual scenes, restoring the scenes' state after each trial.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import time
+
+from . import _abi
+
+DEFAULT_OPTS = dict(graph=True, graph_substeps=1, vertex_block=256, grid_pct=100,
+                    xcd_split=True)
+
+
+def _opts_struct(opts: dict) -> _abi.XpbdBatchOpts:
+    o = _abi.XpbdBatchOpts()
+    _abi.check(_abi.load().xpbd_batch_opts_init(ctypes.byref(o)))
+    o.graph = int(bool(opts.get("graph", True)))
+    o.graph_substeps = int(opts.get("graph_substeps", 1))
+    o.vertex_block = int(opts.get("vertex_block", 256))
+    o.grid_pct = int(opts.get("grid_pct", 100))
+    o.xcd_split = int(bool(opts.get("xcd_split", True)))
+    return o
+
+
+class SceneBatch:
+    """Member scenes stepped together (one substep or frame of every member)."""
+
+    def __init__(self, solvers, **opts):
+        self._lib = _abi.load()
+        self.solvers = list(solvers)
+        if not self.solvers:
+            raise ValueError("a batch needs at least one scene")
+        self.opts = dict(DEFAULT_OPTS)
+        self.opts.update(opts)
+        for s in self.solvers:
+            s._sync_params()
+            s.synchronize()
+        arr = (ctypes.c_void_p * len(self.solvers))(*[s._h for s in self.solvers])
+        h = ctypes.c_void_p()
+        o = _opts_struct(self.opts)
+        _abi.check(self._lib.xpbd_batch_create(arr, len(self.solvers), ctypes.byref(o),
+                                               ctypes.byref(h)))
+        self._h = h.value
+        ns = ctypes.c_int32()
+        nv = ctypes.c_int64()
+        nt = ctypes.c_int64()
+        npch = ctypes.c_int32()
+        _abi.check(self._lib.xpbd_batch_info(self._h, ctypes.byref(ns), ctypes.byref(nv),
+                                             ctypes.byref(nt), ctypes.byref(npch)))
+        self.n_scenes, self.n_verts, self.n_tets, self.n_patches = (ns.value, nv.value, nt.value,
+                                                                     npch.value)
+        s0 = self.solvers[0]
+        self.iters = int(s0.constraints_iter)
+        self.substeps_num = int(s0.substeps_num)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.xpbd_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def substep(self, n: int = 1):
+        """n substeps of every member (asynchronous)."""
+        for _ in range(n):
+            _abi.check(self._lib.xpbd_batch_substep(self._h))
+
+    def frame(self, n: int = 1):
+        """n frames of every member (asynchronous); member counters follow."""
+        _abi.check(self._lib.xpbd_batch_frame(self._h, int(n)))
+
+    def synchronize(self):
+        _abi.check(self._lib.xpbd_batch_synchronize(self._h))
+        for s in self.solvers:
+            s._refresh_counters()
+
+    @property
+    def tet_iterations_per_substep(self) -> int:
+        return self.n_tets * self.iters
+
+
+def _snapshot(solvers):
+    return [(s.mesh.verts.x.to_numpy(), s.mesh.verts.v.to_numpy(), s.rho_hat, s.current_frame,
+             s.time) for s in solvers]
+
+
+def _restore(solvers, snap):
+    for s, (x, v, rho, frame, t) in zip(solvers, snap):
+        s.mesh.verts.x.from_numpy(x)
+        s.mesh.verts.v.from_numpy(v)
+        s.current_frame = frame
+        s.time = t
+        _abi.check(s._lib.xpbd_set_state(s._h, int(frame), float(t), float(rho)))
+
+
+def default_candidates(substeps_num: int):
+    grid = itertools.product([128, 256, 512], [50, 100], [True, False],
+                             sorted({1, int(substeps_num)}))
+    return [dict(graph=True, vertex_block=vb, grid_pct=gp, xcd_split=xs, graph_substeps=gs)
+            for vb, gp, xs, gs in grid]
+
+
+def autotune(solvers, candidates=None, substeps: int | None = None, warmup: int = 1):
+    """Times each candidate option set on the scenes and returns
+    (best_opts, [(opts, seconds_per_substep), ...]).  Scene state (positions,
+    velocities, rho_hat, counters) is restored after every trial."""
+    solvers = list(solvers)
+    s0 = solvers[0]
+    if candidates is None:
+        candidates = default_candidates(s0.substeps_num)
+    snap = _snapshot(solvers)
+    results = []
+    try:
+        for opts in candidates:
+            per = int(opts.get("graph_substeps", 1))
+            n = substeps if substeps is not None else max(2, per)
+            n = max(per, (n // per) * per)
+            with SceneBatch(solvers, **opts) as b:
+                b.substep(max(warmup, 0) * per)
+                b.synchronize()
+                t0 = time.perf_counter()
+                b.substep(n)
+                b.synchronize()
+                el = (time.perf_counter() - t0) / n
+            results.append((dict(opts), el))
+            _restore(solvers, snap)
+    finally:
+        _restore(solvers, snap)
+    best = min(results, key=lambda r: r[1])[0]
+    return best, results
+
+
+def run_sequential(solvers, frames: int = 1):
+    """The same scenes solved one after another with plain per-scene solves."""
+    for s in solvers:
+        for _ in range(frames):
+            s.solve()
+
+
+def scene_bytes(solver) -> int:
+    b = ctypes.c_int64()
+    _abi.check(_abi.load().xpbd_get_device_bytes(solver._h, ctypes.byref(b)))
+    return int(b.value)
+
+
+__all__ = ["SceneBatch", "autotune", "default_candidates", "run_sequential", "scene_bytes",
+           "DEFAULT_OPTS"]

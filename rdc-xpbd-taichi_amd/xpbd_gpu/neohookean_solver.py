@@ -298,6 +298,17 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         self.time = t.value
         self.current_frame = frame.value
 
+    def _refresh_counters(self):
+        """Reads current_frame / current_substep / time back from the device
+        solver (after frames stepped by a batch)."""
+        frame = _abi.ctypes.c_int32()
+        substep = _abi.ctypes.c_int32()
+        t = ctypes_double()
+        _abi.check(self._lib.xpbd_get_state(self._h, _ref(frame), _ref(substep), _ref(t), None))
+        self.current_frame = frame.value
+        self.current_substep = substep.value
+        self.time = t.value
+
     def _substeps_per_frame(self) -> int:
         left = self.frame_dt
         chunks = 0

@@ -116,6 +116,30 @@ def cfg3_stress(n=32, seed=0, iters=200) -> Scene:
     return s
 
 
+def batched_scenes(n_scenes: int = 256, seed: int = 0, n_range=(6, 40), cell: float = 1.0 / 32.0,
+                   substeps: int = 5, iters: int = 50, drop: float = 0.05) -> list:
+    """The batched configuration: n_scenes boxes whose resolution per axis is
+    drawn uniformly from n_range (independently per axis), 6 tets per hex, cell
+    size `cell`, each dropped from `drop` above the shared plane y=0; E
+    log-uniform in [1e4, 1e7], nu uniform in [0.3, 0.49], density 1000;
+    dt = 1/60, 5 substeps x 50 iterations."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_scenes):
+        nx, ny, nz = (int(v) for v in rng.integers(n_range[0], n_range[1] + 1, size=3))
+        E = float(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
+        nu = float(rng.uniform(0.3, 0.49))
+        size = (nx * cell, ny * cell, nz * cell)
+        m = box_grid(nx, ny, nz, size=size, origin=(0.0, drop, 0.0))
+        vol = size[0] * size[1] * size[2]
+        out.append(Scene(f"batch{i}", m, [plane(0.0)],
+                         base_params(young_modulus=E, poisson_ratio=nu, substeps_num=substeps,
+                                     constraints_iter=iters),
+                         total_mass=1000.0 * vol, frames=1,
+                         extra=dict(dims=(nx, ny, nz))))
+    return out
+
+
 def make_solver(scene: Scene, **opts):
     from .neohookean_solver import XPBDNeoHookeanSolver
     from .sdf import PlaneSdf, SphereSdf

@@ -1,0 +1,91 @@
+"""Batched form: one set of launches for many scenes, checked against the same
+scenes alone (bit-exact, deterministic mode) and against the oracle."""
+import numpy as np
+import pytest
+
+from scenes import batched_scenes, cfg1_contact, make_oracle, make_solver
+from xpbd_gpu.batch import SceneBatch, autotune
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} {kw}", flush=True)
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(graph=False), dict(graph_substeps=5),
+                                  dict(xcd_split=False, vertex_block=128, grid_pct=50)])
+def test_batch_members_match_alone_bitwise(opts):
+    _print_case("batch_vs_alone", **opts)
+    scenes = batched_scenes(n_scenes=6, seed=3, n_range=(3, 9), iters=20)
+    alone = [make_solver(s) for s in scenes]
+    mixed = [make_solver(s) for s in scenes]
+    for s in alone:
+        s.solve()
+        s.solve()
+    with SceneBatch(mixed, **opts) as b:
+        b.frame(2)
+        b.synchronize()
+    for a, m in zip(alone, mixed):
+        assert np.array_equal(a.mesh.verts.x.to_numpy(), m.mesh.verts.x.to_numpy())
+        assert np.array_equal(a.mesh.verts.v.to_numpy(), m.mesh.verts.v.to_numpy())
+        assert a.rho_hat == m.rho_hat
+        assert m.current_frame == 2
+
+
+def test_batch_member_matches_oracle():
+    _print_case("batch_vs_oracle")
+    scenes = batched_scenes(n_scenes=4, seed=5, n_range=(3, 8), iters=30)
+    solvers = [make_solver(s) for s in scenes]
+    with SceneBatch(solvers) as b:
+        b.frame(3)
+        b.synchronize()
+    for sc, s in zip(scenes, solvers):
+        o = make_oracle(sc)
+        o.frame(3)
+        err = np.abs(s.mesh.verts.x.to_numpy() - o.x).max()
+        assert err <= 1e-4 * sc.diag, (sc.name, err)
+
+
+def test_batch_substeps_and_mixed_sizes():
+    _print_case("batch_substeps")
+    scenes = batched_scenes(n_scenes=3, seed=7, n_range=(2, 12), iters=10) + [cfg1_contact(n=4)]
+    scenes[-1].params.update(substeps_num=5, constraints_iter=10)
+    solvers = [make_solver(s) for s in scenes]
+    ref = [make_solver(s) for s in scenes]
+    with SceneBatch(solvers) as b:
+        b.substep(3)
+        b.synchronize()
+        assert all(s.current_substep == 2 for s in solvers)
+        b.substep(2)
+        b.frame(1)
+        b.synchronize()
+    for s in ref:
+        s.solve()
+        s.solve()
+    for a, m in zip(ref, solvers):
+        assert np.array_equal(a.mesh.verts.x.to_numpy(), m.mesh.verts.x.to_numpy())
+
+
+def test_batch_rejects_mismatched_members():
+    a = make_solver(cfg1_contact(n=2))
+    sc = cfg1_contact(n=2)
+    sc.params["constraints_iter"] = 7
+    b = make_solver(sc)
+    with pytest.raises(ValueError):
+        SceneBatch([a, b])
+    with pytest.raises(ValueError):
+        SceneBatch([a, a])
+
+
+def test_autotune_restores_state():
+    _print_case("autotune")
+    scenes = batched_scenes(n_scenes=4, seed=9, n_range=(3, 8), iters=10)
+    solvers = [make_solver(s) for s in scenes]
+    x0 = [s.mesh.verts.x.to_numpy() for s in solvers]
+    best, res = autotune(solvers, candidates=[dict(vertex_block=128), dict(vertex_block=256),
+                                              dict(graph_substeps=5)])
+    assert best in [r[0] for r in res] and len(res) == 3
+    for s, x in zip(solvers, x0):
+        assert np.array_equal(s.mesh.verts.x.to_numpy(), x)
+        assert s.current_frame == 0
