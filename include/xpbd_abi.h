@@ -168,7 +168,7 @@ int xpbd_clear_log(xpbd_solver* s);
  * (neohookean_solver.py:55-56, 65; xpbd_base.py:27). */
 int xpbd_get_state(xpbd_solver* s, int32_t* frame, int32_t* substep, double* time,
                    double* rho_hat);
-/* rho_hat = NaN keeps the device value. */
+/* Sets the counters at a frame boundary; rho_hat = NaN keeps the device value. */
 int xpbd_set_state(xpbd_solver* s, int32_t frame, double time, double rho_hat);
 
 /* Size of the mesh as stored on the device (after padding). */

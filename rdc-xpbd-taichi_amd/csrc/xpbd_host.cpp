@@ -364,6 +364,7 @@ struct xpbd_solver {
   bool table_valid = false;
 
   int frame = 0, substep = 0;
+  int sub_pos = 0;  // substeps of the current frame already run by a batch
   double time = 0.0;
   bool logging = false;
   std::vector<double> log_rows;
@@ -1330,6 +1331,8 @@ int xpbd_set_state(xpbd_solver* s, int32_t frame, double time, double rho_hat) {
   }
   s->frame = frame;
   s->time = time;
+  s->substep = 0;
+  s->sub_pos = 0;  // a restored state starts at a frame boundary
   return XPBD_OK;
 }
 
@@ -1475,7 +1478,6 @@ struct xpbd_batch {
   SceneDev proto{};  // flags and iteration count shared by the members
   hipGraphExec_t gexec = nullptr;
   int graph_substeps = 0;
-  int pos_in_frame = 0;  // substeps of the current frame already run
 
   ~xpbd_batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1530,19 +1532,22 @@ int batch_enqueue_substep(xpbd_batch* b) {
   return XPBD_OK;
 }
 
+// Per-member frame accounting: members may join a batch mid-frame.
 void batch_count_substep(xpbd_batch* b) {
   for (xpbd_solver* s : b->scenes) {
-    s->substep = b->pos_in_frame;
-  }
-  b->pos_in_frame += 1;
-  const xpbd_desc& d = b->scenes[0]->prm;
-  if (b->pos_in_frame == d.substeps_num) {
-    b->pos_in_frame = 0;
-    for (xpbd_solver* s : b->scenes) {
+    s->substep = s->sub_pos;
+    if (++s->sub_pos == s->prm.substeps_num) {
+      s->sub_pos = 0;
       s->time += s->prm.dt;
       s->frame += 1;
     }
   }
+}
+
+bool batch_at_frame_start(const xpbd_batch* b) {
+  for (const xpbd_solver* s : b->scenes)
+    if (s->sub_pos != 0) return false;
+  return true;
 }
 
 int batch_run(xpbd_batch* b, int n_substeps) {
@@ -1557,7 +1562,7 @@ int batch_run(xpbd_batch* b, int n_substeps) {
   const int per = b->opts.graph_substeps;
   int done = 0;
   while (done < n_substeps) {
-    const int chunk = (per > 1 && n_substeps - done >= per && b->pos_in_frame == 0) ? per : 1;
+    const int chunk = (per > 1 && n_substeps - done >= per && batch_at_frame_start(b)) ? per : 1;
     if (!b->gexec || b->graph_substeps != chunk) {
       if (b->gexec) {
         HIP_TRY(hipGraphExecDestroy(b->gexec));
@@ -1673,7 +1678,6 @@ int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_b
   TRY(b->table.alloc(n_scenes));
   TRY(batch_sync_table(b.get()));
   b->proto = b->table_host[0];
-  b->pos_in_frame = 0;
   *out = b.release();
   return XPBD_OK;
 }
@@ -1700,8 +1704,8 @@ int xpbd_batch_frame(xpbd_batch* b, int32_t n_frames) {
   // frame_dt a frame is exactly substeps_num substeps
   if (d.dt != d.frame_dt)
     return set_error(XPBD_EINVAL, "batch frames need dt == frame_dt; step substeps instead");
-  if (b->pos_in_frame != 0)
-    return set_error(XPBD_ESTATE, "a frame is partly stepped; finish it with xpbd_batch_substep");
+  if (!batch_at_frame_start(b))
+    return set_error(XPBD_ESTATE, "a member's frame is partly stepped; finish it with substeps");
   return batch_run(b, n_frames * d.substeps_num);
 }
 

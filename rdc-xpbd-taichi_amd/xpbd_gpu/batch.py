@@ -117,6 +117,7 @@ def _restore(solvers, snap):
 def default_candidates(substeps_num: int):
     grid = itertools.product([128, 256, 512], [50, 100], [True, False],
                              sorted({1, int(substeps_num)}))
+    # graph_substeps > 1 holds whole frames (dt == frame_dt)
     return [dict(graph=True, vertex_block=vb, grid_pct=gp, xcd_split=xs, graph_substeps=gs)
             for vb, gp, xs, gs in grid]
 
@@ -137,10 +138,12 @@ def autotune(solvers, candidates=None, substeps: int | None = None, warmup: int 
             n = substeps if substeps is not None else max(2, per)
             n = max(per, (n // per) * per)
             with SceneBatch(solvers, **opts) as b:
-                b.substep(max(warmup, 0) * per)
+                # whole-frame graphs are replayed by frame(); single substeps by substep()
+                step = (lambda k: b.frame(k // per)) if per > 1 else b.substep
+                step(max(warmup, 0) * per)
                 b.synchronize()
                 t0 = time.perf_counter()
-                b.substep(n)
+                step(n)
                 b.synchronize()
                 el = (time.perf_counter() - t0) / n
             results.append((dict(opts), el))

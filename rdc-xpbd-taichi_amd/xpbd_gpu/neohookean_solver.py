@@ -298,6 +298,23 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         self.time = t.value
         self.current_frame = frame.value
 
+    def save_state(self) -> dict:
+        """Host checkpoint of everything a later substep reads: x, v, rho_hat
+        (persists across substeps, neohookean_solver.py:65), frame counter and
+        time.  Multipliers and Chebyshev buffers are rebuilt every substep."""
+        self.synchronize()
+        self._refresh_counters()
+        return {"x": self.mesh.verts.x.to_numpy(), "v": self.mesh.verts.v.to_numpy(),
+                "rho_hat": self.rho_hat, "current_frame": self.current_frame, "time": self.time}
+
+    def load_state(self, state: dict):
+        self.mesh.verts.x.from_numpy(state["x"])
+        self.mesh.verts.v.from_numpy(state["v"])
+        self.current_frame = int(state["current_frame"])
+        self.time = float(state["time"])
+        _abi.check(self._lib.xpbd_set_state(self._h, self.current_frame, self.time,
+                                            float(state["rho_hat"])))
+
     def _refresh_counters(self):
         """Reads current_frame / current_substep / time back from the device
         solver (after frames stepped by a batch)."""
