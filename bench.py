@@ -61,6 +61,109 @@ def cpu_baseline(scene, substeps: int, threads: int) -> dict:
                       f"OpenMP over tets"}
 
 
+def batched_measurement(n_scenes: int, frames: int, tune: bool) -> dict:
+    """256 ragged scenes in one batched launch sequence vs the same scenes
+    solved one after another with plain per-scene solves."""
+    from xpbd_gpu.batch import SceneBatch, autotune, default_candidates
+    from xpbd_gpu.scenes import batched_scenes, make_solver
+
+    scenes = batched_scenes(n_scenes=n_scenes)
+    t0 = time.perf_counter()
+    solvers = [make_solver(s) for s in scenes]
+    setup = time.perf_counter() - t0
+    T = sum(s.n_tets for s in solvers)
+    iters = scenes[0].params["constraints_iter"]
+    subs = scenes[0].params["substeps_num"]
+    print(f"[bench] batched: {len(solvers)} scenes, {T} tets, setup {setup:.1f} s",
+          file=sys.stderr, flush=True)
+    opts, trials = {}, []
+    if tune:
+        cands = [c for c in default_candidates(subs) if c["grid_pct"] == 100]
+        opts, trials = autotune(solvers, candidates=cands)
+    with SceneBatch(solvers, **opts) as b:
+        b.frame(1)  # warm-up (graph capture)
+        b.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            s0 = time.perf_counter()
+            b.frame(1)
+            b.synchronize()
+            lat.append(time.perf_counter() - s0)
+        el = time.perf_counter() - t0
+    batched = T * iters * subs * frames / el
+    # sequential plain solves of the same scenes, one frame each
+    for s in solvers[:4]:
+        s.solve()  # warm-up of per-scene graphs
+    seq_lat = []
+    t0 = time.perf_counter()
+    for s in solvers:
+        s0 = time.perf_counter()
+        s.solve()
+        seq_lat.append(time.perf_counter() - s0)
+    seq_el = time.perf_counter() - t0
+    sequential = T * iters * subs / seq_el
+    return {
+        "workload": f"{len(solvers)} box scenes, per-axis resolution uniform in 6..40 (seed 0), "
+                    f"{T} tets in total, {subs} substeps x {iters} iterations per frame",
+        "tet_iterations_per_s": batched,
+        "sequential_plain_tet_iterations_per_s": sequential,
+        "speedup_vs_sequential": batched / sequential,
+        "frame_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                     "p99": float(np.percentile(lat, 99) * 1e3)},
+        "sequential_frame_ms_per_scene": {"p50": float(np.percentile(seq_lat, 50) * 1e3),
+                                          "p99": float(np.percentile(seq_lat, 99) * 1e3)},
+        "sequential_all_scenes_ms": seq_el * 1e3,
+        "tuned_opts": opts,
+        "tuning_trials": len(trials),
+    }
+
+
+def served_measurement(n_scenes: int, rate: float, seconds: float) -> dict:
+    """Poisson request stream over resident scenes: continuous batching vs the
+    same trace served one request at a time with plain solves."""
+    from xpbd_gpu.serve import Server, poisson_trace, run_sequential, served_scenes, summarize
+
+    scenes = served_scenes(n_scenes=n_scenes)
+    reqs = poisson_trace(n_scenes, rate=rate, duration=seconds, seed=1)
+    srv = Server(scenes, hbm_budget=200e9)
+    srv.warm(range(n_scenes))
+    warm = [r for r in poisson_trace(n_scenes, rate=rate, duration=0.2, seed=7)]
+    srv.run(warm)
+    t0 = time.perf_counter()
+    done = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    served = summarize(done, wall)
+    print(f"[bench] served: {len(reqs)} requests in {wall:.1f} s, p99 {served['p99_ms']:.1f} ms",
+          file=sys.stderr, flush=True)
+    seq_solvers = {}
+    for k in range(n_scenes):
+        from xpbd_gpu.scenes import make_solver
+        seq_solvers[k] = make_solver(scenes[k])
+        seq_solvers[k].solve()  # warm-up of per-scene graphs
+    t0 = time.perf_counter()
+    sdone = run_sequential(scenes, reqs, solvers=seq_solvers)
+    swall = time.perf_counter() - t0
+    seq = summarize(sdone, swall)
+    tets = np.array([s.mesh.n_tets for s in scenes])
+    return {
+        "workload": f"{n_scenes} resident scenes, tet counts log-normal median 2e4 in [1e3, 5e5] "
+                    f"(seed 0; min {tets.min()}, max {tets.max()}), Poisson {rate:.0f} req/s over "
+                    f"{seconds:.0f} s (seed 1), 1-4 frames per request, HBM budget 200 GB",
+        "requests": served["requests"],
+        "p50_step_latency_ms": served["p50_ms"],
+        "p99_step_latency_ms": served["p99_ms"],
+        "tet_iterations_per_s": served["tet_iterations_per_s"],
+        "sequential_plain": {"p50_step_latency_ms": seq["p50_ms"],
+                             "p99_step_latency_ms": seq["p99_ms"],
+                             "tet_iterations_per_s": seq["tet_iterations_per_s"],
+                             "makespan_s": seq["makespan_s"]},
+        "speedup_vs_sequential": served["tet_iterations_per_s"] / seq["tet_iterations_per_s"],
+        "batch_rebuilds": srv.rebuilds,
+        "substeps": srv.substeps_run,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +177,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--roofline-frames", type=int, default=1)
+    ap.add_argument("--batched-scenes", type=int, default=256)
+    ap.add_argument("--batched-frames", type=int, default=3)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--serve-scenes", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--plain-only", action="store_true",
+                    help="skip the batched and served measurements")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
     args = ap.parse_args()
@@ -163,6 +274,12 @@ def main():
                             "p99": float(np.percentile(lat, 99) * 1e3)},
         "roofline": roofline,
     }
+    del solver
+    if not args.plain_only:
+        out["batched"] = batched_measurement(args.batched_scenes, args.batched_frames,
+                                             not args.no_tune)
+        out["served"] = served_measurement(args.serve_scenes, args.serve_rate,
+                                           args.serve_seconds)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps, threads)

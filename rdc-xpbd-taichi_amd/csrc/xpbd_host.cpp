@@ -266,8 +266,15 @@ void build_patches(const std::vector<int>& tets, int nv, int nt, Patches& P,
 // and `cuts` the end of each piece.
 void rcb_order(const std::vector<float>& cent, const int* tets, int nt, int nv,
                std::vector<int>& order, std::vector<int>& cuts) {
-  order.resize(nt);
-  std::iota(order.begin(), order.end(), 0);
+  struct Rec {
+    float c[3];
+    int id;
+  };
+  // records are partitioned in place, so every pass streams through memory
+  std::vector<Rec> rec(nt);
+  for (int t = 0; t < nt; ++t) {
+    rec[t] = {{cent[3 * size_t(t)], cent[3 * size_t(t) + 1], cent[3 * size_t(t) + 2]}, t};
+  }
   cuts.clear();
   std::vector<int> stamp(nv, -1), lcount(nv, 0);
   int stamp_id = 0;
@@ -276,7 +283,7 @@ void rcb_order(const std::vector<float>& cent, const int* tets, int nt, int nv,
     int nverts = 0;
     for (int i = lo; i < hi; ++i)
       for (int k = 0; k < 4; ++k) {
-        const int g = tets[4 * size_t(order[i]) + k];
+        const int g = tets[4 * size_t(rec[i].id) + k];
         if (stamp[g] != stamp_id) {
           stamp[g] = stamp_id;
           lcount[g] = 0;
@@ -288,20 +295,19 @@ void rcb_order(const std::vector<float>& cent, const int* tets, int nt, int nv,
   };
   std::vector<int2> stack;
   if (nt > 0) stack.push_back(make_int2(0, nt));
-  std::vector<int2> leaves;
   while (!stack.empty()) {
     const int2 r = stack.back();
     stack.pop_back();
     const int lo = r.x, hi = r.y, n = hi - lo;
     if (n <= kPatchTets && leaf_ok(lo, hi)) {
-      leaves.push_back(r);
+      cuts.push_back(hi);
       continue;
     }
     float mn[3] = {1e30f, 1e30f, 1e30f}, mx[3] = {-1e30f, -1e30f, -1e30f};
     for (int i = lo; i < hi; ++i)
       for (int k = 0; k < 3; ++k) {
-        mn[k] = std::min(mn[k], cent[3 * size_t(order[i]) + k]);
-        mx[k] = std::max(mx[k], cent[3 * size_t(order[i]) + k]);
+        mn[k] = std::min(mn[k], rec[i].c[k]);
+        mx[k] = std::max(mx[k], rec[i].c[k]);
       }
     int axis = 0;
     for (int k = 1; k < 3; ++k)
@@ -313,13 +319,14 @@ void rcb_order(const std::vector<float>& cent, const int* tets, int nt, int nv,
     } else {
       m = lo + n / 2;
     }
-    std::nth_element(order.begin() + lo, order.begin() + m, order.begin() + hi,
-                     [&](int a, int b) { return cent[3 * size_t(a) + axis] < cent[3 * size_t(b) + axis]; });
+    std::nth_element(rec.begin() + lo, rec.begin() + m, rec.begin() + hi,
+                     [axis](const Rec& a, const Rec& b) { return a.c[axis] < b.c[axis]; });
     // depth first, low half first: push high then low
     stack.push_back(make_int2(m, hi));
     stack.push_back(make_int2(lo, m));
   }
-  for (const int2& l : leaves) cuts.push_back(l.y);
+  order.resize(nt);
+  for (int t = 0; t < nt; ++t) order[t] = rec[t].id;
 }
 
 }  // namespace
