@@ -27,6 +27,14 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBPS = 8000.0
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_patch_project_cfg2_pmc.txt")
+PMC_COMMANDS = """\
+cd <repo root>; export TMPDIR=/tmp; D=<output folder>
+rocprofv3 --pmc FETCH_SIZE --output-format csv -d $D/fetch -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline --roofline-frames 0 --plain-only
+rocprofv3 --pmc WRITE_SIZE --output-format csv -d $D/write -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline --roofline-frames 0 --plain-only
+rocprofv3 --pmc FETCH_SIZE --output-format csv -d $D/cfetch -o run -- python3 bench.py --copy-calibration-mib 512
+rocprofv3 --pmc WRITE_SIZE --output-format csv -d $D/cwrite -o run -- python3 bench.py --copy-calibration-mib 512
+python3 bench.py --summarize-pmc $D"""
 
 
 def b_alg_iteration(T: int, V: int) -> int:
@@ -40,6 +48,103 @@ def b_alg_project(T: int, V: int) -> int:
     (inv_Dm) + 4 (V0) + 8 + 8 (lambda read/write); per vertex 16 (x, inv_m
     gather) + 12 (delta_x write)."""
     return 72 * T + 28 * V
+
+
+def _pmc_per_kernel(path: str) -> dict:
+    """{kernel short name: [counter value per dispatch, ...]} from a rocprofv3
+    counter_collection.csv (values in KiB for FETCH_SIZE / WRITE_SIZE)."""
+    import csv
+    out = {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+            out.setdefault(name, []).append(float(r["Counter_Value"]))
+    return out
+
+
+def summarize_pmc(folder: str, copy_mib: int = 512) -> str:
+    """Per-dispatch HBM-side traffic of every kernel of one cfg2 frame from the
+    FETCH_SIZE and WRITE_SIZE passes, each scaled by what the same counter
+    reads for a plain device copy of known size (the copy's own kernel)."""
+    f = _pmc_per_kernel(os.path.join(folder, "fetch", "run_counter_collection.csv"))
+    w = _pmc_per_kernel(os.path.join(folder, "write", "run_counter_collection.csv"))
+    cf = _pmc_per_kernel(os.path.join(folder, "cfetch", "run_counter_collection.csv"))
+    cw = _pmc_per_kernel(os.path.join(folder, "cwrite", "run_counter_collection.csv"))
+    copy_kib = copy_mib * 1024.0
+    ck = next(k for k in cf if "copyBuffer" in k)
+    f_read = float(np.mean(cf[ck]))
+    w_read = float(np.mean(cw[ck]))
+    f_scale, w_scale = copy_kib / f_read, copy_kib / w_read
+    lines = ["# HBM-side traffic per dispatch of the cfg2 frame (64^3 jittered box, 1 572 864 tets,",
+             "# 274 625 vertices, 10 substeps x 100 iterations), rocprofv3 FETCH_SIZE / WRITE_SIZE.",
+             "# Made by:"]
+    lines += ["#   " + c for c in PMC_COMMANDS.splitlines()]
+    lines += [f"copy calibration: {ck}, {copy_mib} MiB each way per launch;",
+              f"  FETCH_SIZE reads {f_read:.1f} KiB -> scale {f_scale:.4f};"
+              f" WRITE_SIZE reads {w_read:.1f} KiB -> scale {w_scale:.4f}",
+              "",
+              f"{'kernel':28s} {'dispatches':>10s} {'FETCH KiB':>12s} {'WRITE KiB':>12s}"
+              f" {'bytes/dispatch (scaled)':>24s}"]
+    rows = []
+    for k in sorted(set(f) | set(w)):
+        if not k.startswith("xpbd::"):
+            continue
+        fv = float(np.mean(f.get(k, [0.0])))
+        wv = float(np.mean(w.get(k, [0.0])))
+        b = (fv * f_scale + wv * w_scale) * 1024.0
+        rows.append((b * len(f.get(k, [])), k, len(f.get(k, [])), fv, wv, b))
+    for _, k, n, fv, wv, b in sorted(rows, reverse=True):
+        lines.append(f"{k:28s} {n:10d} {fv:12.1f} {wv:12.1f} {b:24.0f}")
+    proj = next(r for r in rows if r[1] == "xpbd::k_patch_project")
+    lines += ["", f"traffic_bytes_per_launch xpbd::k_patch_project: {proj[5]:.0f}"]
+    return "\n".join(lines) + "\n"
+
+
+STATS_COMMAND = ("cd <repo root>; export TMPDIR=/tmp; rocprofv3 --kernel-trace --stats "
+                 "--output-format csv -d <D> -o run -- python3 bench.py --steps 3 --warmup 1 "
+                 "--no-cpu-baseline --plain-only; python3 bench.py --summarize-stats <D>")
+
+
+def summarize_stats(folder: str) -> str:
+    """Per-kernel launch counts and durations from rocprofv3 --kernel-trace
+    --stats, plus the projection kernel's average over the graph-replayed
+    frames alone (warmup + timed steps, before the event-timed frames)."""
+    import csv
+    with open(os.path.join(folder, "run_kernel_stats.csv")) as fh:
+        stats = list(csv.DictReader(fh))
+    with open(os.path.join(folder, "run_kernel_trace.csv")) as fh:
+        proj = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
+                      for r in csv.DictReader(fh) if "k_patch_project" in r["Kernel_Name"])
+    lines = ["# rocprofv3 kernel statistics of the cfg2 bench (64^3 jittered box, 1 572 864 tets,",
+             "# 274 625 vertices, 10 substeps x 100 iterations per frame): 1 warmup + 3 timed",
+             "# graph frames, then the event-timed frames (8 projection replays per iteration,",
+             "# then one event pair per projection).",
+             "# Made by: " + STATS_COMMAND, "",
+             f"{'kernel':30s} {'calls':>7s} {'avg us':>9s} {'min us':>9s} {'max us':>9s} {'%':>7s}"]
+    for r in stats:
+        name = r["Name"].split("(")[0].replace("void ", "").strip()
+        lines.append(f"{name:30s} {int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:9.2f} "
+                     f"{float(r['MinNs']) / 1e3:9.2f} {float(r['MaxNs']) / 1e3:9.2f} "
+                     f"{float(r['Percentage']):7.2f}")
+    graph = proj[:4000]
+    if graph:
+        avg = sum(e - b for b, e in graph) / len(graph) / 1e3
+        lines += ["", f"xpbd::k_patch_project over the {len(graph)} graph-replayed launches: "
+                      f"avg {avg:.2f} us"]
+    return "\n".join(lines) + "\n"
+
+
+def committed_traffic(path: str = PMC_SUMMARY):
+    """The projection kernel's measured bytes per launch from the committed
+    counter summary, or None when it is absent."""
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch xpbd::k_patch_project:"):
+                    return int(float(line.split(":")[1]))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(scene, substeps: int, threads: int) -> dict:
@@ -181,13 +286,26 @@ def main():
     ap.add_argument("--batched-frames", type=int, default=3)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--serve-scenes", type=int, default=64)
-    ap.add_argument("--serve-rate", type=float, default=100.0)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--plain-only", action="store_true",
                     help="skip the batched and served measurements")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device copy of this size (counter calibration)")
+    ap.add_argument("--summarize-pmc", default="",
+                    help="only summarize the FETCH_SIZE/WRITE_SIZE passes in this folder")
+    ap.add_argument("--summarize-stats", default="",
+                    help="only summarize a rocprofv3 --kernel-trace --stats folder")
+    ap.add_argument("--timing-replays", type=int, default=8,
+                    help="back-to-back projection replays per event pair for the roofline")
     args = ap.parse_args()
+
+    if args.summarize_pmc:
+        sys.stdout.write(summarize_pmc(args.summarize_pmc))
+        return
+    if args.summarize_stats:
+        sys.stdout.write(summarize_stats(args.summarize_stats))
+        return
 
     if args.copy_calibration_mib:
         # A plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE readings
@@ -235,17 +353,26 @@ def main():
     value = tet_iters / elapsed
     gbps = b_alg_iteration(T, V) * iters * subs * args.steps / elapsed / 1e9
 
-    # dominant kernel: HIP events around every projection launch (its stream)
-    solver.set_kernel_timing(True)
-    for _ in range(args.roofline_frames):
-        solver.solve()
-    ms, launches = solver.kernel_time()
-    solver.set_kernel_timing(False)
-    avg_s = ms / 1e3 / max(1, launches)
+    # dominant kernel: HIP events on the solver's stream around runs of
+    # back-to-back projection replays (one event pair per run, so the pair's
+    # own overhead is spread over the run), then around single launches
+    def timed_avg(replays):
+        solver.set_kernel_timing(True, replays=replays)
+        for _ in range(args.roofline_frames):
+            solver.solve()
+        ms, n = solver.kernel_time()
+        solver.set_kernel_timing(False)
+        return ms / 1e3 / max(1, n), n
+
+    avg_s, launches = timed_avg(max(2, args.timing_replays))
+    avg_single_s, _ = timed_avg(1)
     ach = b_alg_project(T, V) / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = committed_traffic() if (args.n == 64 and args.layout == "patch") else None
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": "k_project", "avg_us": round(avg_s * 1e6, 2), "launches": launches,
+                "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                "kernel": "xpbd::k_patch_project" if args.layout == "patch" else "xpbd::k_project",
+                "avg_us": round(avg_s * 1e6, 2), "launches": launches,
+                "avg_us_one_launch_per_event_pair": round(avg_single_s * 1e6, 2),
                 "bytes_per_launch": b_alg_project(T, V)}
 
     out = {

@@ -30,7 +30,7 @@ def build(verbose: bool = True) -> str:
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
     for src in SOURCES:
         obj = os.path.join(HERE, "_build", src + ".o")
-        lang = ["-x", "hip"]
+        lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip", "--offload-host-only"]
         cmd = [HIPCC, *COMMON, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -25,6 +25,20 @@
 
 namespace xpbd {
 
+// Pointers held in the device-side scene table point into global memory.  The
+// kernels read them from that table, so the compiler cannot infer their
+// address space and would emit flat loads, which count against both the
+// vector-memory and the LDS wait counters: every LDS wait would then also wait
+// for the global loads in flight.  The device pass therefore types them as
+// address space 1 (global); the host pass sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define XPBD_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define XPBD_GLOBAL_AS
+#endif
+template <class T>
+using gptr = XPBD_GLOBAL_AS T*;
+
 struct SdfDev {
   int kind;   // XPBD_SDF_PLANE / XPBD_SDF_SPHERE
   int fixed;
@@ -48,36 +62,36 @@ struct SceneCtrl {
 
 struct SceneDev {
   // vertex streams (n_verts)
-  float4* pos;
-  float4* xs;
-  float4* vel;
-  float4* prev;
-  float4* xj;
-  float4* dacc;
-  const float4* rest;  // rest positions (after scale/offset), w unused
-  const float* valence;
+  gptr<float4> pos;
+  gptr<float4> xs;
+  gptr<float4> vel;
+  gptr<float4> prev;
+  gptr<float4> xj;
+  gptr<float4> dacc;
+  gptr<const float4> rest;  // rest positions (after scale/offset), w unused
+  gptr<const float> valence;
   // flat layout: one thread per tet
-  const int* adj_off;
-  const int* adj;
-  float4* corner;
-  const int4* tet;
-  const float4* dm0;
-  const float4* dm1;
-  const float2* dm2;
+  gptr<const int> adj_off;
+  gptr<const int> adj;
+  gptr<float4> corner;
+  gptr<const int4> tet;
+  gptr<const float4> dm0;
+  gptr<const float4> dm1;
+  gptr<const float2> dm2;
   // patch layout: one workgroup per patch of <= 256 tets / <= 256 vertices
-  const int* p_tet_begin;   // n_patches + 1
-  const int* p_vert_begin;  // n_patches + 1: range of the patch's vertex records
-  const int* p_vert_ids;    // per patch vertex: global vertex id
-  const int2* p_vert_sum;   // per patch vertex: (corner base | corner count << 16 |
+  gptr<const int> p_tet_begin;   // n_patches + 1
+  gptr<const int> p_vert_begin;  // n_patches + 1: range of the patch's vertex records
+  gptr<const int> p_vert_ids;    // per patch vertex: global vertex id
+  gptr<const int2> p_vert_sum;   // per patch vertex: (corner base | corner count << 16 |
                             //   owner << 31, slot); the owner patch writes the vertex
-  float4* ybuf[3];          // rotating Chebyshev iterates y^k (ybuf[0] is pos)
-  const uint2* tet_local;   // per tet: 4 x u8 local vertex ids, 4 x u8 corner ranks
-  const int* vslot_off;     // n_verts + 1: each vertex's slots, contiguous, ascending patch
-  float4* slot_acc;         // per slot: partial delta_x (xyz) and constraint count (w)
-  float2* lam;
-  double* partial;  // 3 doubles per tet workgroup / patch: sum r, sum |C_H|, sum |C_D|
-  SceneCtrl* ctrl;
-  double* log;      // 6 doubles per iteration: iter, r_hat, rho_hat, omega, hydro, dev
+  gptr<float4> ybuf[3];          // rotating Chebyshev iterates y^k (ybuf[0] is pos)
+  gptr<const uint2> tet_local;   // per tet: 4 x u8 local vertex ids, 4 x u8 corner ranks
+  gptr<const int> vslot_off;     // n_verts + 1: each vertex's slots, contiguous, ascending patch
+  gptr<float4> slot_acc;         // per slot: partial delta_x (xyz) and constraint count (w)
+  gptr<float2> lam;
+  gptr<double> partial;  // 3 doubles per tet workgroup / patch: sum r, sum |C_H|, sum |C_D|
+  gptr<SceneCtrl> ctrl;
+  gptr<double> log;      // 6 doubles per iteration: iter, r_hat, rho_hat, omega, hydro, dev
   int n_verts;
   int n_tets;
   int n_partials;

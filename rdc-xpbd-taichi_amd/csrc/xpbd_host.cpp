@@ -331,6 +331,11 @@ void rcb_order(const std::vector<float>& cent, const int* tets, int nt, int nv,
 
 }  // namespace
 
+struct TimedSpan {
+  hipEvent_t begin = nullptr, end = nullptr;
+  int launches = 0;
+};
+
 struct xpbd_solver {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -381,8 +386,12 @@ struct xpbd_solver {
   hipGraphExec_t gexec = nullptr;
   int graph_key = -1;
 
-  bool timing = false;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending, ev_free;
+  // kernel timing: 0 off; 1 brackets each projection launch with an event
+  // pair; r > 1 times r back-to-back replays of each projection (lambda saved
+  // before and restored after, so the solve itself is unchanged)
+  int timing = 0;
+  DevBuf<float2> lam_save;
+  std::vector<TimedSpan> ev_pending, ev_free;
   double timed_ms = 0.0;
   int64_t timed_launches = 0;
 
@@ -394,19 +403,19 @@ struct xpbd_solver {
     add(vslot_off); add(p_vert_sum); add(tet_local); add(pitems); add(prange_proj);
     add(prange_res); add(valence); add(adj_off); add(adj); add(d_old2new); add(tet); add(dm0);
     add(dm1); add(dm2); add(lam); add(partial); add(logbuf); add(ctrl); add(table); add(vmap);
-    add(tmap); add(scratch_xyz); add(flag);
+    add(tmap); add(scratch_xyz); add(flag); add(lam_save);
     return b;
   }
 
   ~xpbd_solver() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (auto& e : ev_pending) {
-      (void)hipEventDestroy(e.first);
-      (void)hipEventDestroy(e.second);
+      (void)hipEventDestroy(e.begin);
+      (void)hipEventDestroy(e.end);
     }
     for (auto& e : ev_free) {
-      (void)hipEventDestroy(e.first);
-      (void)hipEventDestroy(e.second);
+      (void)hipEventDestroy(e.begin);
+      (void)hipEventDestroy(e.end);
     }
     if (log_host) (void)hipHostFree(log_host);
     if (stream) (void)hipStreamDestroy(stream);
@@ -634,31 +643,32 @@ int upload_inv_m(xpbd_solver* s) {
   return XPBD_OK;
 }
 
-int events_begin(xpbd_solver* s, std::pair<hipEvent_t, hipEvent_t>& ev) {
+int events_begin(xpbd_solver* s, TimedSpan& ev) {
   if (!s->ev_free.empty()) {
     ev = s->ev_free.back();
     s->ev_free.pop_back();
   } else {
-    HIP_TRY(hipEventCreate(&ev.first));
-    HIP_TRY(hipEventCreate(&ev.second));
+    HIP_TRY(hipEventCreate(&ev.begin));
+    HIP_TRY(hipEventCreate(&ev.end));
   }
-  HIP_TRY(hipEventRecord(ev.first, s->stream));
+  HIP_TRY(hipEventRecord(ev.begin, s->stream));
   return XPBD_OK;
 }
 
-int events_end(xpbd_solver* s, std::pair<hipEvent_t, hipEvent_t>& ev) {
-  HIP_TRY(hipEventRecord(ev.second, s->stream));
+int events_end(xpbd_solver* s, TimedSpan& ev, int launches) {
+  HIP_TRY(hipEventRecord(ev.end, s->stream));
+  ev.launches = launches;
   s->ev_pending.push_back(ev);
   return XPBD_OK;
 }
 
 int events_collect(xpbd_solver* s) {
   for (auto& ev : s->ev_pending) {
-    HIP_TRY(hipEventSynchronize(ev.second));
+    HIP_TRY(hipEventSynchronize(ev.end));
     float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, ev.first, ev.second));
+    HIP_TRY(hipEventElapsedTime(&ms, ev.begin, ev.end));
     s->timed_ms += ms;
-    s->timed_launches += 1;
+    s->timed_launches += ev.launches;
     s->ev_free.push_back(ev);
   }
   s->ev_pending.clear();
@@ -675,16 +685,33 @@ int enqueue_iterations(xpbd_solver* s, bool capturing) {
   const SceneDev& S = s->host_scene;
   const SceneDev* T = s->table.p;
   const bool patch = s->layout == XPBD_LAYOUT_PATCH;
-  for (int it = 0; it < S.iters; ++it) {
-    std::pair<hipEvent_t, hipEvent_t> ev;
-    const bool timed = s->timing && !capturing;
-    if (timed) TRY(events_begin(s, ev));
+  auto project = [&](int it) -> int {
     if (patch)
       HIP_TRY(xpbd::launch_patch_project(T, s->pitems.p, s->prange_proj.p, s->grid_proj, it,
                                          s->stream));
     else
       HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
-    if (timed) TRY(events_end(s, ev));
+    return XPBD_OK;
+  };
+  const size_t lam_bytes = sizeof(float2) * s->lam.n;
+  for (int it = 0; it < S.iters; ++it) {
+    TimedSpan ev;
+    const int reps = capturing ? 0 : s->timing;
+    if (reps > 1) {
+      // The projection only writes lambda, the slot partials and the
+      // iterate buffer of this iteration, from inputs it does not write, so
+      // replays differ from the real launch only in the lambda they leave.
+      HIP_TRY(hipMemcpyAsync(s->lam_save.p, S.lam, lam_bytes, hipMemcpyDeviceToDevice,
+                             s->stream));
+      TRY(events_begin(s, ev));
+      for (int r = 0; r < reps; ++r) TRY(project(it));
+      TRY(events_end(s, ev, reps));
+      HIP_TRY(hipMemcpyAsync(S.lam, s->lam_save.p, lam_bytes, hipMemcpyDeviceToDevice,
+                             s->stream));
+    }
+    if (reps == 1) TRY(events_begin(s, ev));
+    TRY(project(it));
+    if (reps == 1) TRY(events_end(s, ev, 1));
     if (patch)
       HIP_TRY(xpbd::launch_patch_average(T, s->vmap.p, s->nvb, s->bs_v, it, s->stream));
     else
@@ -1360,8 +1387,11 @@ int xpbd_set_exec_path(xpbd_solver* s, int32_t path) {
 
 int xpbd_set_kernel_timing(xpbd_solver* s, int32_t enabled) {
   CHECK_HANDLE(s);
+  if (enabled < 0 || enabled > 1000)
+    return set_error(XPBD_EINVAL, "kernel timing mode %d outside [0, 1000]", enabled);
   TRY(events_collect(s));
-  s->timing = enabled != 0;
+  if (enabled > 1 && s->lam_save.n < s->lam.n) TRY(s->lam_save.alloc(s->lam.n));
+  s->timing = enabled;
   s->timed_ms = 0.0;
   s->timed_launches = 0;
   return XPBD_OK;

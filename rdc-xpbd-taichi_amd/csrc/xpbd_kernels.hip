@@ -347,7 +347,9 @@ __device__ __forceinline__ float4 cheb_blend(const SceneDev& S, float4 y4, float
 __device__ __forceinline__ float4 stage_iterate(const SceneDev& S, int g, int it, bool owner) {
   if (!S.fused || it == 0) return S.pos[g];
   const float4 y = S.ybuf[(it - 1) % 3][g];
-  const float4 pr = it == 1 ? y : S.ybuf[(it - 2) % 3][g];
+  // prev is y^(it-2), or y^(it-1) itself at it == 1; selecting the buffer (not
+  // the value) keeps both loads global
+  const float4 pr = S.ybuf[it == 1 ? 0 : (it - 2) % 3][g];
   const float4 out = cheb_blend(S, y, S.xj[g], pr, float(S.ctrl->omega));
   if (owner) S.ybuf[it % 3][g] = out;
   return out;
@@ -738,7 +740,7 @@ __global__ void k_final_blend(const SceneDev* __restrict__ scenes,
   if (v >= S.n_verts || n_iter <= 0) return;
   const int k = n_iter - 1;
   const float4 y = S.ybuf[k % 3][v];
-  const float4 pr = k == 0 ? y : S.ybuf[(k - 1) % 3][v];
+  const float4 pr = S.ybuf[k == 0 ? 0 : (k - 1) % 3][v];  // y itself at k == 0
   S.pos[v] = cheb_blend(S, y, S.xj[v], pr, float(S.ctrl->omega));
 }
 

@@ -335,8 +335,13 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         return chunks * self.substeps_num
 
     # ---- execution options ----------------------------------------------------
-    def set_kernel_timing(self, enabled: bool):
-        _abi.check(self._lib.xpbd_set_kernel_timing(self._h, int(bool(enabled))))
+    def set_kernel_timing(self, enabled, replays: int = 1):
+        """Times the tet projection kernel with HIP events on the solver's
+        stream.  replays == 1 brackets every projection launch; replays > 1
+        times that many back-to-back replays of each projection (one event
+        pair per run, lambda restored afterwards, so the solve is unchanged)."""
+        mode = 0 if not enabled else max(1, int(replays))
+        _abi.check(self._lib.xpbd_set_kernel_timing(self._h, mode))
 
     def kernel_time(self):
         ms = ctypes_double()

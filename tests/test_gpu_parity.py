@@ -171,6 +171,28 @@ def test_graph_matches_kernels():
     assert np.array_equal(a.mesh.verts.x.to_numpy(), b.mesh.verts.x.to_numpy())
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("replays", [1, 4])
+def test_kernel_timing_leaves_results_unchanged(layout, replays):
+    """Timed projection replays restore lambda, so a timed solve is bit-identical
+    to an untimed one, and the timer counts every timed launch."""
+    _print_case("kernel_timing", layout=layout, replays=replays)
+    scene = cfg1_contact(n=8, frames=2)
+    a = make_solver(scene, accumulate_mode="csr", layout=layout, log_residuals=True)
+    b = make_solver(scene, accumulate_mode="csr", layout=layout, log_residuals=True)
+    b.set_kernel_timing(True, replays=replays)
+    for _ in range(scene.frames):
+        a.solve()
+        b.solve()
+    ms, launches = b.kernel_time()
+    b.set_kernel_timing(False)
+    p = scene.params
+    assert launches == replays * scene.frames * p["substeps_num"] * p["constraints_iter"]
+    assert ms > 0.0
+    assert np.array_equal(a.mesh.verts.x.to_numpy(), b.mesh.verts.x.to_numpy())
+    assert np.array_equal(residual_rows(a), residual_rows(b))
+
+
 def test_unreordered_matches_oracle_closely():
     """Without internal reordering the flat CSR gather sums in the oracle's tet order."""
     _print_case("no_reorder")
