@@ -89,8 +89,9 @@ struct SceneDev {
   gptr<const int> vslot_off;     // n_verts + 1: each vertex's slots, contiguous, ascending patch
   gptr<float4> slot_acc;         // per slot: partial delta_x (xyz) and constraint count (w)
   gptr<float2> lam;
-  gptr<double> partial;  // 3 doubles per tet workgroup / patch: sum r, sum |C_H|, sum |C_D|
-  gptr<SceneCtrl> ctrl;
+  gptr<double> partial;  // per tet workgroup / patch: sum r, sum |C_H|, sum |C_D| (SoA,
+                         // component c at [c * n_partials])
+  gptr<SceneCtrl> ctrl;   // two slots, see the controller update in the kernels
   gptr<double> log;      // 6 doubles per iteration: iter, r_hat, rho_hat, omega, hydro, dev
   int n_verts;
   int n_tets;
@@ -105,6 +106,9 @@ struct SceneDev {
   float gamma;
   float hydro_relax, dev_relax;
   float decay;      // exp(-dt * damping), evaluated once on the host in f32
+  // compliance scales: alpha~ = 1 / (lame V0 dt^2) = alpha_k / det(Dm), with
+  // alpha_k = 6 / (lame dt^2) (deviatoric: lame2, hydrostatic: lame1)
+  float alpha_kd, alpha_kh;
   float g[3];
   int block_mode;   // block_neohookean_enable
   int cheb;         // cheb_enable
@@ -135,6 +139,12 @@ struct __align__(32) PatchItem {
 };
 
 constexpr float kEps = 1e-12f;  // xpbd_gpu/constants.py:3 as an f32 constant
+
+// Residual partial sums of the patch layout: one per group of kPartialGroup
+// consecutive patches of a scene (scene-local patch index / kPartialGroup), so
+// a scene's sums do not depend on how work is split or which scenes share a
+// batch.  The residual kernel's work ranges start at group boundaries.
+constexpr int kPartialGroup = 8;
 
 // Hot-path division and square root on the native gfx950 instructions
 // (v_rcp_f32, v_sqrt_f32: about 1 ulp) instead of the correctly rounded
@@ -299,28 +309,18 @@ __device__ __forceinline__ float3 sdf_project(const SdfDev& s, float3 x, float3 
   return x;
 }
 
-// V0 and inv_Dm of a tet from its rest positions, the formulas of the set-up
-// (calc_tet_volume, xpbd_base.py:6-8; Matrix.inverse of cols(x1-x0, x2-x0,
-// x3-x0), neohookean_solver.py:102-107), recomputed per use instead of
-// streaming 40 bytes per tet; they agree with the stored f32 values to a few
-// ulp.
+// inv_Dm and 1/det(Dm) of a tet from its rest positions (initialize,
+// neohookean_solver.py:97-107; V0 = det(Dm) / 6 as calc_tet_volume,
+// xpbd_base.py:6-8), recomputed where used.  With Dm = cols(a, b, c) the
+// rows of inv_Dm are (b x c, c x a, a x b) / det, det = a . (b x c).
 __device__ __forceinline__ void rest_shape(float3 p0, float3 p1, float3 p2, float3 p3, Mat3& inv,
-                                           float& V0) {
+                                           float& inv_det) {
   const float3 a = p1 - p0, b = p2 - p0, c = p3 - p0;
-  const float3 ab = make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-  V0 = (ab.x * c.x + ab.y * c.y + ab.z * c.z) * (1.f / 6.f);
-  const float m[3][3] = {{a.x, b.x, c.x}, {a.y, b.y, c.y}, {a.z, b.z, c.z}};
-  const float det = m[0][0] * (m[1][1] * m[2][2] - m[2][1] * m[1][2]) -
-                    m[1][0] * (m[0][1] * m[2][2] - m[2][1] * m[0][2]) +
-                    m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
-  const float inv_det = frcp(det);
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-      inv.m[j][i] = inv_det * (m[i1][j1] * m[i2][j2] - m[i2][j1] * m[i1][j2]);
-    }
+  const float3 bc = cross(b, c), ca = cross(c, a), ab = cross(a, b);
+  inv_det = frcp(dot(a, bc));
+  inv.m[0][0] = bc.x * inv_det; inv.m[0][1] = bc.y * inv_det; inv.m[0][2] = bc.z * inv_det;
+  inv.m[1][0] = ca.x * inv_det; inv.m[1][1] = ca.y * inv_det; inv.m[1][2] = ca.z * inv_det;
+  inv.m[2][0] = ab.x * inv_det; inv.m[2][1] = ab.y * inv_det; inv.m[2][2] = ab.z * inv_det;
 }
 
 }  // namespace xpbd

@@ -447,6 +447,9 @@ void fill_scene_params(xpbd_solver* s, SceneDev& S, double dt, double damping) {
   S.lame1 = float(la);
   S.lame2 = float(mu);
   S.dt = float(dt);
+  // alpha~ = 1 / (lame V0 dt^2) = alpha_k / det(Dm) with V0 = det(Dm) / 6
+  S.alpha_kd = float(6.0 / (double(S.lame2) * double(S.dt) * double(S.dt)));
+  S.alpha_kh = float(6.0 / (double(S.lame1) * double(S.dt) * double(S.dt)));
   S.gamma = float(d.cheb_gamma);
   S.hydro_relax = float(d.hydrostatic_relaxation);
   S.dev_relax = float(d.deviatoric_relaxation);
@@ -499,7 +502,9 @@ int build_scene(xpbd_solver* s) {
   S.n_tets = s->nt;
   S.layout = s->layout;
   S.n_patches = s->n_patches;
-  S.n_partials = s->layout == XPBD_LAYOUT_PATCH ? s->n_patches : s->ntb;
+  S.n_partials = s->layout == XPBD_LAYOUT_PATCH
+                     ? (s->n_patches + xpbd::kPartialGroup - 1) / xpbd::kPartialGroup
+                     : s->ntb;
   S.tet_block = s->bs_t;
   fill_scene_params(s, S, s->prm.dt / s->prm.substeps_num, s->prm.damping);
   s->table_valid = false;
@@ -570,6 +575,30 @@ int xcd_ranges(int n_items, int n_blocks, int n_xcd, std::vector<int2>& ranges) 
     }
   for (char c : seen)
     if (!c) return set_error(XPBD_ESTATE, "work split misses an item");
+  return XPBD_OK;
+}
+
+// Residual work ranges: as xcd_ranges, over units that each hold one group of
+// kPartialGroup consecutive patches of one scene, so every range starts at a
+// group boundary and each group's partial is summed by one workgroup in order.
+int grouped_ranges(const std::vector<PatchItem>& items, int n_blocks, int n_xcd,
+                   std::vector<int2>& ranges, int* n_used) {
+  std::vector<int> starts;
+  for (size_t i = 0; i < items.size(); ++i)
+    if (i == 0 || items[i].scene != items[i - 1].scene || items[i].patch % xpbd::kPartialGroup == 0)
+      starts.push_back(int(i));
+  const int n_units = int(starts.size());
+  const int nb = std::max(1, std::min(n_blocks, n_units));
+  std::vector<int2> ur;
+  TRY(xcd_ranges(n_units, nb, n_xcd, ur));
+  ranges.resize(ur.size());
+  const int n_items = int(items.size());
+  for (size_t b = 0; b < ur.size(); ++b) {
+    const int lo = ur[b].x < n_units ? starts[ur[b].x] : n_items;
+    const int hi = ur[b].y < n_units ? starts[ur[b].y] : n_items;
+    ranges[b] = make_int2(lo, hi);
+  }
+  *n_used = nb;
   return XPBD_OK;
 }
 
@@ -677,9 +706,10 @@ int events_collect(xpbd_solver* s) {
 
 // The Jacobi iterations of solve_step (neohookean_solver.py:474-549).
 //  fused (patch layout, Chebyshev on): per iteration a projection (which
-//    first forms the previous iteration's Chebyshev blend), the Jacobi
-//    average, a residual and the scalar rho/omega update; a last vertex pass
-//    forms the final blend.
+//    first evaluates the previous iteration's rho/omega update and forms its
+//    Chebyshev blend), the Jacobi average and a residual; after the last
+//    iteration the standalone rho/omega update and a vertex pass forming the
+//    final blend.
 //  otherwise: projection, Jacobi average, [residual, rho/omega], [blend].
 int enqueue_iterations(xpbd_solver* s, bool capturing) {
   const SceneDev& S = s->host_scene;
@@ -722,7 +752,8 @@ int enqueue_iterations(xpbd_solver* s, bool capturing) {
                                             s->stream));
       else
         HIP_TRY(xpbd::launch_residual(T, s->tmap.p, s->ntb, s->bs_t, s->stream));
-      HIP_TRY(xpbd::launch_reduce(T, 1, it, s->stream));
+      // fused: the next projection evaluates this iteration's controller update
+      if (!S.fused || it == S.iters - 1) HIP_TRY(xpbd::launch_reduce(T, 1, it, s->stream));
     }
     if (S.cheb && !S.fused) HIP_TRY(xpbd::launch_blend(T, s->vmap.p, s->nvb, s->bs_v, s->stream));
   }
@@ -1040,7 +1071,7 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
     s->grid_res = std::max(1, std::min(s->n_patches, n_cu * std::max(1, occ_r)));
     std::vector<int2> rp, rr;
     TRY(xcd_ranges(s->n_patches, s->grid_proj, xcd_count(), rp));
-    TRY(xcd_ranges(s->n_patches, s->grid_res, xcd_count(), rr));
+    TRY(grouped_ranges(items, s->grid_res, xcd_count(), rr, &s->grid_res));
     TRY(s->prange_proj.upload(rp.data(), rp.size()));
     TRY(s->prange_res.upload(rr.data(), rr.size()));
   }
@@ -1066,7 +1097,8 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
   std::memset(&c0, 0, sizeof(c0));
   c0.rho_hat = 0.999;  // neohookean_solver.py:65
   c0.omega = 1.0;
-  TRY(s->ctrl.upload(&c0, 1));
+  const SceneCtrl c2[2] = {c0, c0};  // two slots (double-buffered controller state)
+  TRY(s->ctrl.upload(c2, 2));
   TRY(s->logbuf.alloc(6 * size_t(std::max(1, d->constraints_iter))));
   TRY(s->logbuf.zero());
   HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&s->log_host),
@@ -1348,6 +1380,7 @@ int xpbd_get_state(xpbd_solver* s, int32_t* frame, int32_t* substep, double* tim
   if (rho_hat) {
     SceneCtrl c;
     HIP_TRY(hipStreamSynchronize(s->stream));
+    // between substeps both controller slots hold the same state
     HIP_TRY(hipMemcpy(&c, s->ctrl.p, sizeof(c), hipMemcpyDeviceToHost));
     *rho_hat = c.rho_hat;
   }
@@ -1361,7 +1394,8 @@ int xpbd_set_state(xpbd_solver* s, int32_t frame, double time, double rho_hat) {
   HIP_TRY(hipMemcpy(&c, s->ctrl.p, sizeof(c), hipMemcpyDeviceToHost));
   if (!std::isnan(rho_hat)) {
     c.rho_hat = rho_hat;
-    HIP_TRY(hipMemcpy(s->ctrl.p, &c, sizeof(c), hipMemcpyHostToDevice));
+    const SceneCtrl c2[2] = {c, c};
+    HIP_TRY(hipMemcpy(s->ctrl.p, c2, sizeof(c2), hipMemcpyHostToDevice));
   }
   s->frame = frame;
   s->time = time;
@@ -1559,7 +1593,7 @@ int batch_enqueue_substep(xpbd_batch* b) {
     if (P.cheb || P.monitor) {
       HIP_TRY(xpbd::launch_patch_residual(T, b->pitems.p, b->prange_res.p, b->grid_res,
                                           b->stream));
-      HIP_TRY(xpbd::launch_reduce(T, ns, it, b->stream));
+      if (!P.fused || it == P.iters - 1) HIP_TRY(xpbd::launch_reduce(T, ns, it, b->stream));
     }
     if (P.cheb && !P.fused) HIP_TRY(xpbd::launch_blend(T, b->vmap.p, b->nvb, bs, b->stream));
   }
@@ -1709,7 +1743,7 @@ int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_b
   b->grid_res = int(std::max(1LL, std::min<long long>(b->n_items, full_r * o.grid_pct / 100)));
   std::vector<int2> rp, rr;
   TRY(xcd_ranges(b->n_items, b->grid_proj, nx, rp));
-  TRY(xcd_ranges(b->n_items, b->grid_res, nx, rr));
+  TRY(grouped_ranges(items, b->grid_res, nx, rr, &b->grid_res));
   TRY(b->prange_proj.upload(rp.data(), rp.size()));
   TRY(b->prange_res.upload(rr.data(), rr.size()));
   TRY(b->table.alloc(n_scenes));

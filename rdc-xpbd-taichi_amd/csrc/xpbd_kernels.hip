@@ -54,41 +54,39 @@ struct TetProjection {
 //  split mode: solve_hydrostatic (:196-254) then solve_deviatoric (:144-194)
 //              on the same positions.
 __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4], const Mat3& idm,
-                                            float V0, float2& lam, TetProjection& out) {
+                                            float inv_det, float2& lam, TetProjection& out) {
   const float w[4] = {q[0].w, q[1].w, q[2].w, q[3].w};
   const Mat3 F = deformation_gradient(f3(q[0]), f3(q[1]), f3(q[2]), f3(q[3]), idm);
-  const float dt = S.dt;
+  // alpha~ = 1 / (lame V0 dt^2) = alpha_k / det(Dm)
+  const float aD = S.alpha_kd * inv_det;
+  const float aH = S.alpha_kh * inv_det;
   if (S.block_mode) {
     const float C_D = norm_sqr(F) - 3.f;
     const float C_H = det3(F) - 1.f;
-    Mat3 gfd;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * 2.f;
-    float3 gD[4], gH[4];
-    vertex_grads(gfd, idm, gD);
+    // grad C_D = 2 F inv_Dm^T: the gradients gF of F inv_Dm^T are formed and
+    // the factor 2 is applied to the scalars below (exact in floating point)
+    float3 gF[4], gH[4];
+    vertex_grads(F, idm, gF);
     vertex_grads(ddet_dF(F), idm, gH);
-    float A11 = w[0] * dot(gD[0], gD[0]) + w[1] * dot(gD[1], gD[1]) + w[2] * dot(gD[2], gD[2]) +
-                w[3] * dot(gD[3], gD[3]);
-    float A22 = w[0] * dot(gH[0], gH[0]) + w[1] * dot(gH[1], gH[1]) + w[2] * dot(gH[2], gH[2]) +
-                w[3] * dot(gH[3], gH[3]);
-    const float A12 = w[0] * dot(gD[0], gH[0]) + w[1] * dot(gD[1], gH[1]) +
-                      w[2] * dot(gD[2], gH[2]) + w[3] * dot(gD[3], gH[3]);
-    const float inv_dt2 = frcp(dt * dt);
-    const float aD = frcp(S.lame2 * V0) * inv_dt2;
-    const float aH = frcp(S.lame1 * V0) * inv_dt2;
-    A11 += aD;
-    A22 += aH;
+    const float sFF = w[0] * dot(gF[0], gF[0]) + w[1] * dot(gF[1], gF[1]) +
+                      w[2] * dot(gF[2], gF[2]) + w[3] * dot(gF[3], gF[3]);
+    const float sHH = w[0] * dot(gH[0], gH[0]) + w[1] * dot(gH[1], gH[1]) +
+                      w[2] * dot(gH[2], gH[2]) + w[3] * dot(gH[3], gH[3]);
+    const float sFH = w[0] * dot(gF[0], gH[0]) + w[1] * dot(gF[1], gH[1]) +
+                      w[2] * dot(gF[2], gH[2]) + w[3] * dot(gF[3], gH[3]);
+    const float A11 = 4.f * sFF + aD;
+    const float A22 = sHH + aH;
+    const float A12 = 2.f * sFH;
     const float b1 = -(C_D + aD * lam.x);
     const float b2 = -(C_H + aH * lam.y);
     float dlD, dlH;
     solve2x2_with_fallback(A11, A12, A22, b1, b2, dlD, dlH);
     lam.x += dlD;
     lam.y += dlH;
+    const float dlD2 = 2.f * dlD;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      out.dp[k] = w[k] * (gD[k] * dlD + gH[k] * dlH);
+      out.dp[k] = w[k] * (gF[k] * dlD2 + gH[k] * dlH);
       out.cnt[k] = 1.f;
     }
   } else {
@@ -102,8 +100,7 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
     float dlH = 0.f;
     const int okH = wsH > kEps;
     if (okH) {
-      const float at = frcp(S.lame1 * V0) * frcp(dt * dt);
-      dlH = fdiv(-(C_H + at * lam.y), wsH + at);
+      dlH = fdiv(-(C_H + aH * lam.y), wsH + aH);
       dlH *= S.hydro_relax;
       lam.y += dlH;
     }
@@ -121,8 +118,7 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
     float dlD = 0.f;
     const int okD = wsD > kEps;
     if (okD) {
-      const float at = frcp(S.lame2 * V0) * frcp(dt * dt);
-      dlD = fdiv(-(C_D + at * lam.x), wsD + at);
+      dlD = fdiv(-(C_D + aD * lam.x), wsD + aD);
       dlD *= S.dev_relax;
       lam.x += dlD;
     }
@@ -140,17 +136,15 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
 // compute_xpbd_residual (neohookean_solver.py:298-329) for one tet, plus the
 // |C_H| and |C_D| of the monitor kernels (:256-296).
 __device__ __forceinline__ void residual_tet(const SceneDev& S, float3 x0, float3 x1, float3 x2,
-                                             float3 x3, const Mat3& idm, float V0, float2 lam,
-                                             float& r, float& ah, float& ad) {
+                                             float3 x3, const Mat3& idm, float inv_det,
+                                             float2 lam, float& r, float& ah, float& ad) {
   const Mat3 F = deformation_gradient(x0, x1, x2, x3, idm);
   const float C_D = norm_sqr(F) - 3.f;
   const float C_H = det3(F) - 1.f;
   r = 0.f;
   if (S.cheb) {
-    const float dt = S.dt;
-    const float inv_dt2 = frcp(dt * dt);
-    const float aD = frcp(S.lame2 * V0) * inv_dt2;
-    const float aH = frcp(S.lame1 * V0) * inv_dt2;
+    const float aD = S.alpha_kd * inv_det;
+    const float aH = S.alpha_kh * inv_det;
     const float rD = C_D + aD * lam.x;
     const float rH = C_H + aH * lam.y;
     r = fsqrt(rD * rD + rH * rH);
@@ -159,13 +153,19 @@ __device__ __forceinline__ void residual_tet(const SceneDev& S, float3 x0, float
   ad = fabsf(C_D);
 }
 
-// Workgroup sum of (r, |C_H|, |C_D|) in a fixed order, written to partial[idx].
-__device__ __forceinline__ void store_partial(const SceneDev& S, int idx, float r, float ah,
-                                              float ad) {
+// Workgroup sum of (r, |C_H|, |C_D|) in a fixed order: f32 wave sums, then
+// the waves in order in double; thread 0 gets the totals.  |C_H| and |C_D| are
+// only summed when `all3` (convergence monitor on; uniform).  The caller puts
+// a workgroup barrier between two calls.
+__device__ __forceinline__ void block_sums(float r, float ah, float ad, bool all3, double tot[3]) {
   __shared__ float red[3][16];
   r = wave_sum(r);
-  ah = wave_sum(ah);
-  ad = wave_sum(ad);
+  if (all3) {
+    ah = wave_sum(ah);
+    ad = wave_sum(ad);
+  } else {
+    ah = ad = 0.f;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
   if (lane == 0) {
     red[0][wave] = r;
@@ -180,11 +180,18 @@ __device__ __forceinline__ void store_partial(const SceneDev& S, int idx, float 
       s1 += red[1][i];
       s2 += red[2][i];
     }
-    double* part = S.partial + 3 * size_t(idx);
-    part[0] = s0;
-    part[1] = s1;
-    part[2] = s2;
+    tot[0] = s0;
+    tot[1] = s1;
+    tot[2] = s2;
   }
+}
+
+// Stores partial idx (component c at partial[c * n_partials + idx]).
+__device__ __forceinline__ void store_partial(const SceneDev& S, int idx, const double tot[3]) {
+  const size_t n = size_t(S.n_partials);
+  S.partial[idx] = tot[0];
+  S.partial[n + idx] = tot[1];
+  S.partial[2 * n + idx] = tot[2];
 }
 
 // x_J = y + delta_x / cons_num for one vertex (post_solve,
@@ -255,7 +262,7 @@ __global__ void k_project(const SceneDev* __restrict__ scenes, const WorkItem* _
   const Mat3 idm = load_inv_dm(S.dm0[t], S.dm1[t], d2);
   float2 lam = first ? make_float2(0.f, 0.f) : S.lam[t];
   TetProjection P;
-  project_tet(S, q, idm, d2.y, lam, P);
+  project_tet(S, q, idm, frcp(6.f * d2.y), lam, P);
   S.lam[t] = lam;
   if (S.csr) {
     float4* c = S.corner + 4 * size_t(t);
@@ -320,10 +327,13 @@ __global__ void k_residual(const SceneDev* __restrict__ scenes, const WorkItem* 
     const int4 id = S.tet[t];
     const float2 d2 = S.dm2[t];
     const Mat3 idm = load_inv_dm(S.dm0[t], S.dm1[t], d2);
-    residual_tet(S, f3(X[id.x]), f3(X[id.y]), f3(X[id.z]), f3(X[id.w]), idm, d2.y, S.lam[t], r,
+    residual_tet(S, f3(X[id.x]), f3(X[id.y]), f3(X[id.z]), f3(X[id.w]), idm, frcp(6.f * d2.y),
+                 S.lam[t], r,
                  ah, ad);
   }
-  store_partial(S, wi.first / S.tet_block, r, ah, ad);
+  double tot[3];
+  block_sums(r, ah, ad, S.monitor != 0, tot);
+  if (threadIdx.x == 0) store_partial(S, wi.first / S.tet_block, tot);
 }
 
 // ---- patch layout ----------------------------------------------------------------
@@ -344,15 +354,147 @@ __device__ __forceinline__ float4 cheb_blend(const SceneDev& S, float4 y4, float
 // x_J^(it-1) (xj), prev = y^(it-2) (or y^0 when it == 1) and omega^(it-1), and
 // the vertex's owner patch stores it in ybuf[it%3]; the other patches holding
 // the vertex compute the same bits.
-__device__ __forceinline__ float4 stage_iterate(const SceneDev& S, int g, int it, bool owner) {
+__device__ __forceinline__ float4 stage_iterate(const SceneDev& S, int g, int it, bool owner,
+                                                float omega) {
   if (!S.fused || it == 0) return S.pos[g];
   const float4 y = S.ybuf[(it - 1) % 3][g];
   // prev is y^(it-2), or y^(it-1) itself at it == 1; selecting the buffer (not
   // the value) keeps both loads global
   const float4 pr = S.ybuf[it == 1 ? 0 : (it - 2) % 3][g];
-  const float4 out = cheb_blend(S, y, S.xj[g], pr, float(S.ctrl->omega));
+  const float4 out = cheb_blend(S, y, S.xj[g], pr, omega);
   if (owner) S.ybuf[it % 3][g] = out;
   return out;
+}
+
+// ---- per-scene scalar control ----------------------------------------------------
+
+// Workgroup size of every scene reduction, so that the standalone reduce and
+// the one folded into the projection sum in the same order.
+constexpr int kReduceThreads = kPatchThreads;
+
+// Fixed-order sums of a scene's partials (component 0, and 1-2 when `all3`)
+// by one workgroup of kReduceThreads threads: each thread adds its strided
+// entries in ascending order, then waves and workgroup combine in a fixed
+// tree.  Every workgroup that calls it gets the same bits; thread 0 gets the
+// totals.  Ends with a workgroup barrier.
+__device__ __forceinline__ void scene_sums(const SceneDev& S, bool all3, double tot[3]) {
+  __shared__ double red[3][kReduceThreads / 64];
+  const int n = S.n_partials;
+  const int tid = threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (c > 0 && !all3) break;
+    gptr<const double> p = S.partial + size_t(c) * size_t(n);
+    double sum = 0.0;
+    int b = tid;
+    // four independent loads in flight per step
+    for (; b + 3 * kReduceThreads < n; b += 4 * kReduceThreads) {
+      const double a0 = p[b], a1 = p[b + kReduceThreads], a2 = p[b + 2 * kReduceThreads],
+                   a3 = p[b + 3 * kReduceThreads];
+      sum += a0;
+      sum += a1;
+      sum += a2;
+      sum += a3;
+    }
+    for (; b < n; b += kReduceThreads) sum += p[b];
+    acc[c] = wave_sum_d(sum);
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  if (lane == 0) {
+    red[0][wave] = acc[0];
+    red[1][wave] = acc[1];
+    red[2][wave] = acc[2];
+  }
+  __syncthreads();
+  if (tid == 0) {
+    tot[0] = tot[1] = tot[2] = 0.0;
+    for (int w = 0; w < kReduceThreads / 64; ++w) {
+      tot[0] += red[0][w];
+      tot[1] += red[1][w];
+      tot[2] += red[2][w];
+    }
+  }
+  __syncthreads();
+}
+
+// The residual-driven rho estimate and Chebyshev omega after iteration it
+// (neohookean_solver.py:461-466, 492-504, 537-546), in double, from the
+// controller state after iteration it-1 and the scene's residual sums; also
+// forms the iteration's log row (iter, r_hat, rho_hat, omega, hydro, dev).
+__device__ __forceinline__ SceneCtrl ctrl_update(const SceneDev& S, SceneCtrl c, int it,
+                                                 const double tot[3], double L[6]) {
+  if (it == 0) {  // prev_r_hat = None and omega = 1.0 per substep (:461, :466)
+    c.have_prev = 0;
+    c.omega = 1.0;
+  }
+  const double cnt = double(S.n_tets);
+  double r_hat = 0.0;
+  double omega = 1.0;
+  double rho = c.rho_hat;
+  if (S.cheb) {
+    r_hat = S.n_tets > 0 ? tot[0] / cnt : 0.0;
+    if (c.have_prev && c.prev_r_hat > 1e-12) {
+      double rho_inst = r_hat / (c.prev_r_hat + 1e-12);
+      rho_inst = fmax(0.00001, fmin(0.99999, rho_inst));
+      rho = (1.0 - 0.2) * rho + 0.2 * rho_inst;
+      c.rho_hat = rho;
+    }
+    c.prev_r_hat = r_hat;
+    c.have_prev = 1;
+    const double rho2 = rho * rho;
+    if (it < S.warmup)
+      omega = 1.0;
+    else if (it == S.warmup)
+      omega = 2.0 / (2.0 - rho2);
+    else
+      omega = 4.0 / (4.0 - rho2 * c.omega);
+    c.omega = omega;
+    c.r_hat = r_hat;
+  }
+  L[0] = it;
+  L[1] = r_hat;
+  L[2] = rho;
+  L[3] = omega;
+  L[4] = S.monitor && S.n_tets > 0 ? tot[1] / cnt : 0.0;
+  L[5] = S.monitor && S.n_tets > 0 ? tot[2] / cnt : 0.0;
+  return c;
+}
+
+__device__ __forceinline__ void store_log_row(const SceneDev& S, int it, const double L[6]) {
+  gptr<double> dst = S.log + 6 * size_t(it);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) dst[k] = L[k];
+}
+
+// The controller state is double-buffered: the update after iteration k reads
+// ctrl[(k + 1) & 1] and writes ctrl[k & 1].  In the fused sequence every
+// projection workgroup of iteration it evaluates the update after it-1 for
+// its scenes itself (identical bits in all of them), so the separate reduce
+// launch is needed only after the last iteration; the workgroup holding the
+// scene's patch 0 stores the new state and the log row, into the slot no
+// workgroup of this launch reads.  Returns omega^(it-1) to every thread.
+__device__ __forceinline__ float scene_omega(const SceneDev& S, int it, bool writer) {
+  __shared__ float s_omega;
+  double tot[3];
+  scene_sums(S, S.monitor != 0, tot);
+  if (threadIdx.x == 0) {
+    double L[6];
+    const SceneCtrl c = ctrl_update(S, S.ctrl[it & 1], it - 1, tot, L);
+    s_omega = float(c.omega);
+    if (writer) {
+      S.ctrl[(it - 1) & 1] = c;
+      store_log_row(S, it - 1, L);
+    }
+  }
+  __syncthreads();
+  return s_omega;
+}
+
+// Out-of-line form for the projection's scene changes inside its loop (batches
+// only), which keeps the loop's register use down.
+__device__ __noinline__ float fused_omega(const SceneDev& S, int it, bool writer) {
+  return scene_omega(S, it, writer);
 }
 
 // Persistent patch projection.  Workgroup b walks the patch items
@@ -384,18 +526,30 @@ __global__ void __launch_bounds__(kPatchThreads)
   float4 y0 = make_float4(0.f, 0.f, 0.f, 0.f), r0 = y0;
   uint2 loc0 = make_uint2(0u, 0u);
   float2 lam0 = make_float2(0.f, 0.f);
-  {
-    const SceneDev& S = scenes[h0.scene];
-    if (tid < h0.n_verts) {
-      const int g = S.p_vert_ids[h0.vert_begin + tid];
-      vs0 = S.p_vert_sum[h0.vert_begin + tid];
-      y0 = stage_iterate(S, g, it, vs0.x < 0);
-      r0 = S.rest[g];
+  // omega^(it-1) of the scene whose patches are being staged
+  float omega = 1.f;
+  int omega_scene = -1;
+  // the first patch's loads, the blend inputs included, are issued before the
+  // controller update so that they are in flight while it runs
+  const SceneDev& S0 = scenes[h0.scene];
+  const bool blend0 = S0.fused && !first;
+  int g0 = 0;
+  float4 ya = y0, yb = y0, xa = y0;
+  if (tid < h0.n_verts) {
+    g0 = S0.p_vert_ids[h0.vert_begin + tid];
+    vs0 = S0.p_vert_sum[h0.vert_begin + tid];
+    r0 = S0.rest[g0];
+    if (blend0) {
+      ya = S0.ybuf[(it - 1) % 3][g0];
+      yb = S0.ybuf[it == 1 ? 0 : (it - 2) % 3][g0];
+      xa = S0.xj[g0];
+    } else {
+      ya = S0.pos[g0];
     }
-    if (tid < h0.n_tets) {
-      loc0 = S.tet_local[h0.tet_begin + tid];
-      if (!first) lam0 = S.lam[h0.tet_begin + tid];
-    }
+  }
+  if (tid < h0.n_tets) {
+    loc0 = S0.tet_local[h0.tet_begin + tid];
+    if (!first) lam0 = S0.lam[h0.tet_begin + tid];
   }
   // stage 1 (next): vertex ids only
   bool has1 = rg.x + 1 < rg.y;
@@ -408,6 +562,15 @@ __global__ void __launch_bounds__(kPatchThreads)
     if (tid < h1.n_verts) {
       g1 = S.p_vert_ids[h1.vert_begin + tid];
       vs1 = S.p_vert_sum[h1.vert_begin + tid];
+    }
+  }
+  y0 = ya;
+  if (blend0) {
+    omega = scene_omega(S0, it, h0.patch == 0);
+    omega_scene = h0.scene;
+    if (tid < h0.n_verts) {  // as stage_iterate
+      y0 = cheb_blend(S0, ya, xa, yb, omega);
+      if (vs0.x < 0) S0.ybuf[it % 3][g0] = y0;
     }
   }
   for (int i = rg.x; i < rg.y; ++i) {
@@ -426,8 +589,12 @@ __global__ void __launch_bounds__(kPatchThreads)
     float2 lam1 = make_float2(0.f, 0.f);
     if (has1) {
       const SceneDev& S1 = scenes[h1.scene];
+      if (S1.fused && !first && h1.scene != omega_scene) {  // uniform over the workgroup
+        omega = fused_omega(S1, it, h1.patch == 0);
+        omega_scene = h1.scene;
+      }
       if (tid < h1.n_verts) {
-        y1 = stage_iterate(S1, g1, it, vs1.x < 0);
+        y1 = stage_iterate(S1, g1, it, vs1.x < 0, omega);
         r1 = S1.rest[g1];
       }
       if (tid < h1.n_tets) {
@@ -452,11 +619,12 @@ __global__ void __launch_bounds__(kPatchThreads)
                         int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
       const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
       Mat3 idm;
-      float V0;
-      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
+      float inv_det;
+      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm,
+                 inv_det);
       float2 lam = lam0;
       TetProjection P;
-      project_tet(S, q, idm, V0, lam, P);
+      project_tet(S, q, idm, inv_det, lam, P);
       S.lam[h0.tet_begin + tid] = lam;
       if (csr) {
 #pragma unroll
@@ -527,8 +695,9 @@ __device__ __forceinline__ float4 stage_jacobi(const SceneDev& S, int g) {
 }
 
 // Persistent patch residual, pipelined like k_patch_project: x_J (Chebyshev
-// on) or pos (off) and the rest positions of a patch are staged in LDS, one
-// partial (r, |C_H|, |C_D|) per patch is written to the scene's partials.
+// on) or pos (off) and the rest positions of a patch are staged in LDS; the
+// sums (r, |C_H|, |C_D|) of each group of kPartialGroup patches form one
+// partial of the scene.  Each range starts at a group boundary.
 __global__ void __launch_bounds__(kPatchThreads)
     k_patch_residual(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
                      const int2* __restrict__ ranges) {
@@ -560,6 +729,10 @@ __global__ void __launch_bounds__(kPatchThreads)
     h1 = items[rg.x + 1];
     if (tid < h1.n_verts) g1 = scenes[h1.scene].p_vert_ids[h1.vert_begin + tid];
   }
+  // running sums of the current patch group (thread 0); the range starts at a
+  // group boundary
+  __shared__ double grp[3];
+  if (tid == 0) grp[0] = grp[1] = grp[2] = 0.0;
   for (int i = rg.x; i < rg.y; ++i) {
     const SceneDev& S = scenes[h0.scene];
     if (tid < h0.n_verts) {
@@ -593,12 +766,26 @@ __global__ void __launch_bounds__(kPatchThreads)
       const int l[4] = {int(loc0.x & 0xffu), int((loc0.x >> 8) & 0xffu),
                         int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
       Mat3 idm;
-      float V0;
-      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm, V0);
-      residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, V0, lam0,
+      float inv_det;
+      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm,
+                 inv_det);
+      residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, inv_det, lam0,
                    r, ah, ad);
     }
-    store_partial(S, h0.patch, r, ah, ad);  // ends with a workgroup barrier
+    {
+      double tot[3];
+      block_sums(r, ah, ad, S.monitor != 0, tot);
+      if (tid == 0) {  // running sums of the patch group; stored at its last patch
+        grp[0] += tot[0];
+        grp[1] += tot[1];
+        grp[2] += tot[2];
+        if ((h0.patch + 1) % kPartialGroup == 0 || h0.patch + 1 == S.n_patches) {
+          const double g3[3] = {grp[0], grp[1], grp[2]};
+          store_partial(S, h0.patch / kPartialGroup, g3);
+          grp[0] = grp[1] = grp[2] = 0.0;
+        }
+      }
+    }
     __syncthreads();
     h0 = h1;
     x0 = x1;
@@ -611,90 +798,21 @@ __global__ void __launch_bounds__(kPatchThreads)
   }
 }
 
-// ---- per-scene scalar control ----------------------------------------------------
-
-// One workgroup per scene: fixed-order sum of the scene's partials, then the
-// residual-driven rho estimate and Chebyshev omega of solve_step
-// (neohookean_solver.py:492-504, 537-546) in double.
-__global__ void __launch_bounds__(1024) k_reduce(const SceneDev* __restrict__ scenes, int it) {
+// One workgroup of kReduceThreads per scene: the controller update after
+// iteration it from the scene's partials, stored into both state slots, and
+// the iteration's log row.
+__global__ void __launch_bounds__(kReduceThreads) k_reduce(const SceneDev* __restrict__ scenes,
+                                                           int it) {
   const SceneDev& S = scenes[blockIdx.x];
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  const int n = S.n_partials;
-  const double* part = S.partial;
-  int b = threadIdx.x;
-  // four independent loads in flight per component and step
-  for (; b + 3 * int(blockDim.x) < n; b += 4 * blockDim.x) {
-    double a[4][3];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) a[u][c] = part[3 * size_t(b + u * blockDim.x) + c];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      s0 += a[u][0];
-      s1 += a[u][1];
-      s2 += a[u][2];
-    }
-  }
-  for (; b < n; b += blockDim.x) {
-    s0 += part[3 * size_t(b)];
-    s1 += part[3 * size_t(b) + 1];
-    s2 += part[3 * size_t(b) + 2];
-  }
-  s0 = wave_sum_d(s0);
-  s1 = wave_sum_d(s1);
-  s2 = wave_sum_d(s2);
-  __shared__ double dred[3][16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  if (lane == 0) {
-    dred[0][wave] = s0;
-    dred[1][wave] = s1;
-    dred[2][wave] = s2;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0 || it >= S.iters) return;
-  double tot[3] = {0.0, 0.0, 0.0};
-  for (int i = 0; i < nwave; ++i) {
-    tot[0] += dred[0][i];
-    tot[1] += dred[1][i];
-    tot[2] += dred[2][i];
-  }
-  SceneCtrl* c = S.ctrl;
-  if (it == 0) {  // prev_r_hat = None and omega = 1.0 per substep (:461, :466)
-    c->have_prev = 0;
-    c->omega = 1.0;
-  }
-  const double cnt = double(S.n_tets);
-  double r_hat = 0.0;
-  double omega = 1.0;
-  double rho = c->rho_hat;
-  if (S.cheb) {
-    r_hat = S.n_tets > 0 ? tot[0] / cnt : 0.0;
-    if (c->have_prev && c->prev_r_hat > 1e-12) {
-      double rho_inst = r_hat / (c->prev_r_hat + 1e-12);
-      rho_inst = fmax(0.00001, fmin(0.99999, rho_inst));
-      rho = (1.0 - 0.2) * rho + 0.2 * rho_inst;
-      c->rho_hat = rho;
-    }
-    c->prev_r_hat = r_hat;
-    c->have_prev = 1;
-    const double rho2 = rho * rho;
-    if (it < S.warmup)
-      omega = 1.0;
-    else if (it == S.warmup)
-      omega = 2.0 / (2.0 - rho2);
-    else
-      omega = 4.0 / (4.0 - rho2 * c->omega);
-    c->omega = omega;
-    c->r_hat = r_hat;
-  }
-  double* L = S.log + 6 * size_t(it);
-  L[0] = it;
-  L[1] = r_hat;
-  L[2] = rho;
-  L[3] = omega;
-  L[4] = S.monitor && S.n_tets > 0 ? tot[1] / cnt : 0.0;
-  L[5] = S.monitor && S.n_tets > 0 ? tot[2] / cnt : 0.0;
+  if (it >= S.iters) return;
+  double tot[3];
+  scene_sums(S, true, tot);
+  if (threadIdx.x != 0) return;
+  double L[6];
+  const SceneCtrl c = ctrl_update(S, S.ctrl[(it + 1) & 1], it, tot, L);
+  S.ctrl[0] = c;
+  S.ctrl[1] = c;
+  store_log_row(S, it, L);
 }
 
 // _chebyshev_update + _rotate_prev_with_save (neohookean_solver.py:131-142).
@@ -703,7 +821,7 @@ __global__ void k_blend(const SceneDev* __restrict__ scenes, const WorkItem* __r
   const SceneDev& S = scenes[wi.scene];
   const int v = wi.first + threadIdx.x;
   if (v >= S.n_verts) return;
-  const float omega = float(S.ctrl->omega);
+  const float omega = float(S.ctrl[0].omega);  // both slots hold the update of this iteration
   const float gamma = S.gamma;
   const float4 y4 = S.pos[v];
   const float3 y = f3(y4), xj = f3(S.xj[v]), pr = f3(S.prev[v]);
@@ -720,7 +838,7 @@ __global__ void k_patch_blend(const SceneDev* __restrict__ scenes,
   const SceneDev& S = scenes[wi.scene];
   const int v = wi.first + threadIdx.x;
   if (v >= S.n_verts) return;
-  const float omega = float(S.ctrl->omega);
+  const float omega = float(S.ctrl[0].omega);  // both slots hold the update of this iteration
   const float gamma = S.gamma;
   const float4 xj4 = jacobi_average_slots(S, v, 0);
   const float4 y4 = S.pos[v];
@@ -741,7 +859,8 @@ __global__ void k_final_blend(const SceneDev* __restrict__ scenes,
   const int k = n_iter - 1;
   const float4 y = S.ybuf[k % 3][v];
   const float4 pr = S.ybuf[k == 0 ? 0 : (k - 1) % 3][v];  // y itself at k == 0
-  S.pos[v] = cheb_blend(S, y, S.xj[v], pr, float(S.ctrl->omega));
+  // the last iteration's update comes from the standalone reduce (both slots)
+  S.pos[v] = cheb_blend(S, y, S.xj[v], pr, float(S.ctrl[0].omega));
 }
 
 // advance (xpbd_base.py:148-160).
@@ -839,7 +958,7 @@ hipError_t patch_kernel_occupancy(int* project_blocks_per_cu, int* residual_bloc
                                                       kPatchThreads, 0);
 }
 hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st) {
-  k_reduce<<<n_scenes, 1024, 0, st>>>(scenes, it);
+  k_reduce<<<n_scenes, kReduceThreads, 0, st>>>(scenes, it);
   return hipGetLastError();
 }
 hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,

@@ -1,0 +1,120 @@
+"""Error of the gfx950 solver against the FP64 oracle for every configuration.
+
+For each configuration the device run, the f32 oracle and the f64 oracle start
+from the same state; the report gives, as fractions of the bounding-box
+diagonal, the device's and the f32 oracle's distance to the f64 run and the
+device's distance to the f32 oracle, plus the worst relative deviation of the
+per-iteration residual curve from the f32 oracle's (same gating as the parity
+tests).  The stress case also reports a histogram of det F per tet before and
+after the solve.  Each test appends its entry to parity_report.json in
+$XPBD_REPORT_DIR when that is set, and prints it.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from scenes import cfg1_contact, cfg1_plain_small, cfg2_plain_large, cfg3_stress, make_oracle
+from test_gpu_parity import POS_TOL, check_residual_curve, make_solver
+
+pytestmark = pytest.mark.gpu
+
+THREADS = min(16, os.cpu_count() or 1)
+DET_BINS = [-np.inf, 0.0, 0.1, 0.5, 0.9, 1.1, 2.0, 10.0, np.inf]
+
+
+def _write(name, entry):
+    print(f"[report] {name}: {json.dumps(entry)}", flush=True)
+    d = os.environ.get("XPBD_REPORT_DIR")
+    if not d:
+        return
+    path = os.path.join(d, "parity_report.json")
+    data = {}
+    if os.path.exists(path):
+        with open(path) as fh:
+            data = json.load(fh)
+    data[name] = entry
+    with open(path, "w") as fh:
+        json.dump(data, fh, indent=1)
+
+
+def det_f(x, rest, tets):
+    """det F = det(Ds) / det(Dm) per tet."""
+    def det(p):
+        a = p[tets[:, 1]] - p[tets[:, 0]]
+        b = p[tets[:, 2]] - p[tets[:, 0]]
+        c = p[tets[:, 3]] - p[tets[:, 0]]
+        return np.einsum("ij,ij->i", np.cross(a, b), c)
+    return det(np.asarray(x, np.float64)) / det(np.asarray(rest, np.float64))
+
+
+def histogram(v):
+    h, _ = np.histogram(v, bins=DET_BINS)
+    labels = [f"[{lo:g},{hi:g})" for lo, hi in zip(DET_BINS[:-1], DET_BINS[1:])]
+    return dict(zip(labels, h.tolist()))
+
+
+def run_case(name, scene, frames=None, substeps=None, pos_gate=True, curve=True):
+    print(f"[case] report {name}", flush=True)
+    solver = make_solver(scene, log_residuals=True)
+    o32 = make_oracle(scene, threads=THREADS)
+    o64 = make_oracle(scene, precision=64, threads=THREADS)
+    if substeps is not None:
+        dt = scene.params["dt"] / scene.params["substeps_num"]
+        for _ in range(substeps):
+            solver.apply_external_forces(dt)
+            solver.solve_step(dt)
+            solver.advance(dt, solver.damping)
+            o32.substep(dt)
+            o64.substep(dt)
+    else:
+        for _ in range(frames):
+            solver.solve()
+        o32.frame(frames)
+        o64.frame(frames)
+    x = solver.mesh.verts.x.to_numpy().astype(np.float64)
+    diag = scene.diag
+    e_gpu = float(np.abs(x - o64.x).max() / diag)
+    e_f32 = float(np.abs(o32.x - o64.x).max() / diag)
+    e_pair = float(np.abs(x - o32.x).max() / diag)
+    # free fall keeps every r_hat at round-off, below the curve gate's floors
+    worst = check_residual_curve(solver, o32) if curve else None
+    entry = {"tets": int(scene.mesh.n_tets), "verts": int(scene.mesh.n_verts),
+             "run": f"{frames} frames" if substeps is None else f"{substeps} substeps",
+             "iterations": int(scene.params["constraints_iter"]),
+             "gpu_vs_f64_x_diag": e_gpu, "f32_oracle_vs_f64_x_diag": e_f32,
+             "gpu_vs_f32_oracle_x_diag": e_pair, "residual_curve_worst_rel": worst,
+             "max_r_hat": float(max(r["r_hat"] for r in solver.residual_history))}
+    assert np.all(np.isfinite(x))
+    if pos_gate:
+        assert e_pair <= POS_TOL
+    else:
+        assert e_gpu <= 2.0 * e_f32
+    return solver, entry
+
+
+def test_report_cfg1_as_stated():
+    solver, e = run_case("cfg1", cfg1_plain_small(), frames=10, curve=False)
+    _write("cfg1_free_fall", e)
+
+
+def test_report_cfg1_contact():
+    solver, e = run_case("cfg1_contact", cfg1_contact(), frames=10)
+    _write("cfg1_contact", e)
+
+
+def test_report_cfg2_one_frame():
+    solver, e = run_case("cfg2", cfg2_plain_large(), frames=1)
+    _write("cfg2_one_frame", e)
+
+
+def test_report_cfg3_stress():
+    scene = cfg3_stress()
+    before = det_f(scene.init_x, scene.mesh.verts, scene.mesh.tets)
+    solver, e = run_case("cfg3", scene, frames=1, pos_gate=False)
+    after = det_f(solver.mesh.verts.x.to_numpy(), scene.mesh.verts, scene.mesh.tets)
+    e["det_f_before"] = histogram(before)
+    e["det_f_after"] = histogram(after)
+    e["det_f_after_min_max"] = [float(after.min()), float(after.max())]
+    _write("cfg3_stress", e)
