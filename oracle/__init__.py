@@ -72,6 +72,8 @@ def _load():
     lib.oracle_compute_lumped_mass.argtypes = [vp, ctypes.c_double]
     lib.oracle_rho_hat.argtypes = [vp]
     lib.oracle_rho_hat.restype = ctypes.c_double
+    lib.oracle_set_rho_hat.argtypes = [vp, ctypes.c_double]
+    lib.oracle_set_rho_hat.restype = None
     lib.oracle_set_log.argtypes = [vp, ctypes.c_int]
     lib.oracle_log_size.argtypes = [vp]
     lib.oracle_log_size.restype = ctypes.c_int
@@ -203,6 +205,10 @@ class Oracle:
     @property
     def rho_hat(self) -> float:
         return self._lib.oracle_rho_hat(self._h)
+
+    @rho_hat.setter
+    def rho_hat(self, value: float):
+        self._lib.oracle_set_rho_hat(self._h, float(value))
 
     def set_log(self, enabled: bool):
         self._lib.oracle_set_log(self._h, int(enabled))

@@ -323,6 +323,7 @@ class Solver {
   }
   void set_total_mass(double tm) { total_mass_ = tm; }
   double rho_hat() const { return rho_hat_; }
+  void set_rho_hat(double v) { rho_hat_ = v; }
   double time() const { return time_; }
   int frame() const { return frame_; }
   const std::vector<IterLog>& log() const { return log_; }
@@ -865,6 +866,10 @@ void oracle_compute_lumped_mass(void* hp, double total_mass) {
 }
 double oracle_rho_hat(void* hp) {
   return dispatch(static_cast<Handle*>(hp), [](auto& s) { return s.rho_hat(); });
+}
+// restores a saved rho_hat (a run continued from a stored state)
+void oracle_set_rho_hat(void* hp, double v) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_rho_hat(v); return 0; });
 }
 void oracle_set_log(void* hp, int enabled) {
   dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.log_enabled_ = enabled != 0; return 0; });

@@ -146,12 +146,11 @@ constexpr float kEps = 1e-12f;  // xpbd_gpu/constants.py:3 as an f32 constant
 // batch.  The residual kernel's work ranges start at group boundaries.
 constexpr int kPartialGroup = 8;
 
-// Hot-path division and square root on the native gfx950 instructions
-// (v_rcp_f32, v_sqrt_f32: about 1 ulp) instead of the correctly rounded
-// sequences (a dozen instructions each).  The reference's f32 results are
-// reproduced to the same round-off tolerance as with exact division.
+// Reciprocal and square root on the native gfx950 instructions (v_rcp_f32,
+// v_sqrt_f32: about 1 ulp) instead of the correctly rounded sequences (a
+// dozen instructions each), where the result is not amplified (rest-shape
+// inverse, Jacobi average, residual).  The 2x2 block solve divides exactly.
 __device__ __forceinline__ float frcp(float b) { return __builtin_amdgcn_rcpf(b); }
-__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float fsqrt(float a) { return __builtin_amdgcn_sqrtf(a); }
 
 __device__ __forceinline__ float3 f3(const float4& a) { return make_float3(a.x, a.y, a.z); }
@@ -246,9 +245,15 @@ __device__ __forceinline__ Mat3 ddet_dF(const Mat3& f) {
 }
 
 // solve2x2_pivot_lu (neohookean_solver.py:331-356) with the diagonal fallback
-// of :425-432.
+// of :425-432, in the reference's operation sequence with correctly rounded
+// divisions and no fused multiply-add.  Near F = I the gradients of C_D and
+// C_H are parallel (grad C_D = 2 grad C_H at rest), so for stiff materials the
+// system is nearly singular and `a11 - m * a01` cancels to the compliance
+// terms: the result then depends on the exact rounding of these steps, and a
+// fast reciprocal or a contracted multiply-add changes it by O(1).
 __device__ __forceinline__ void solve2x2_with_fallback(float A11, float A12, float A22, float b1,
                                                        float b2, float& x0, float& x1) {
+#pragma clang fp contract(off)
   float a00 = A11, a01 = A12, a10 = A12, a11 = A22, c0 = b1, c1 = b2;
   int ok = 1;
   if (fabsf(a00) < fabsf(a10)) {
@@ -261,22 +266,21 @@ __device__ __forceinline__ void solve2x2_with_fallback(float A11, float A12, flo
   x0 = 0.f;
   x1 = 0.f;
   if (ok) {
-    const float inv00 = frcp(a00);
-    const float m = a10 * inv00;
+    const float m = a10 / a00;
     const float a11_ = a11 - m * a01;
     const float c1_ = c1 - m * c0;
     if (fabsf(a11_) < kEps) {
       ok = 0;
     } else {
-      x1 = fdiv(c1_, a11_);
-      x0 = (c0 - a01 * x1) * inv00;
+      x1 = c1_ / a11_;
+      x0 = (c0 - a01 * x1) / a00;
     }
   }
   if (!ok) {
     x0 = 0.f;
     x1 = 0.f;
-    if (fabsf(A11) > kEps) x0 = fdiv(b1, A11);
-    if (fabsf(A22) > kEps) x1 = fdiv(b2, A22);
+    if (fabsf(A11) > kEps) x0 = b1 / A11;
+    if (fabsf(A22) > kEps) x1 = b2 / A22;
   }
 }
 

@@ -100,7 +100,7 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
     float dlH = 0.f;
     const int okH = wsH > kEps;
     if (okH) {
-      dlH = fdiv(-(C_H + aH * lam.y), wsH + aH);
+      dlH = -(C_H + aH * lam.y) / (wsH + aH);
       dlH *= S.hydro_relax;
       lam.y += dlH;
     }
@@ -118,7 +118,7 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
     float dlD = 0.f;
     const int okD = wsD > kEps;
     if (okD) {
-      dlD = fdiv(-(C_D + aD * lam.x), wsD + aD);
+      dlD = -(C_D + aD * lam.x) / (wsD + aD);
       dlD *= S.dev_relax;
       lam.x += dlD;
     }

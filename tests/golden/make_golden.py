@@ -1,0 +1,149 @@
+"""Writes the golden fixtures of this directory (one .npz per case).
+
+Each fixture holds a case's inputs (mesh, SDF list, constructor keywords,
+total mass, gravity, optional starting state) and the outputs of the CPU
+oracle on it: final positions and the per-iteration log (frame, substep, iter,
+r_hat, rho_hat, omega, hydro_error, dev_error) in f32 and in f64.
+
+The reference ships no tests and no expected outputs; its only data are the
+TetGen meshes under assets/tetmesh.  The `cow_demo_contact` case takes the cow
+mesh from there and the parameters of the reference's demo driver
+(run_neohookean_usd.py:15-28,82-103: E 5e8, nu 0.4999, dt 0.01 at 60 fps, one
+substep, 200 iterations, fixed plane y=-2 and fixed sphere r=0.3 at (0,-1,0),
+offset (0,1,0), gamma 0.666).  It starts from the state of that run after 54
+frames (as computed by the f32 oracle), just before the cow reaches the
+sphere, and covers the next 6 frames of contact.
+
+Usage (from the repository root; reads the reference checkout only for the
+cow mesh):  python tests/golden/make_golden.py [--reference /root/reference]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+for _p in (ROOT, os.path.join(ROOT, "rdc-xpbd-taichi_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+from oracle import Oracle  # noqa: E402
+from xpbd_gpu.mesh import box_grid, read_tetgen  # noqa: E402
+
+
+def plane(y, fixed):
+    return dict(kind="plane", point=[0.0, y, 0.0], dir=[0.0, 1.0, 0.0], fixed=fixed)
+
+
+def sphere(c, r, fixed):
+    return dict(kind="sphere", center=list(c), radius=r, fixed=fixed)
+
+
+def run_oracle(case, precision, warm_frames=0):
+    o = Oracle(case["verts"], case["tets"], case["sdfs"], precision=precision, threads=8,
+               gravity=case["gravity"], **case["params"])
+    o.compute_lumped_mass(case["total_mass"])
+    if case.get("x0") is not None:
+        o.x = case["x0"]
+    if case.get("v0") is not None:
+        o.v = case["v0"]
+    if case.get("rho0") is not None:
+        o.rho_hat = case["rho0"]
+    o.set_log(True)
+    o.frame(case["frames"])
+    return o.x, o.log()
+
+
+def cases(reference: str):
+    out = {}
+    # one stretched and rotated tet relaxing without gravity
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float32)
+    t = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    c, s = np.cos(0.4), np.sin(0.4)
+    R = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    x0 = (v.astype(np.float64) @ np.diag([1.3, 0.9, 1.1]) @ R.T).astype(np.float32)
+    out["tet1_stretch"] = dict(
+        verts=v, tets=t, x0=x0, sdfs=[plane(-10.0, False)], gravity=(0.0, 0.0, 0.0),
+        total_mass=1.0, frames=2,
+        params=dict(young_modulus=1e5, poisson_ratio=0.3, frame_dt=1 / 60, dt=1 / 60,
+                    substeps_num=2, constraints_iter=20, block_neohookean_enable=True,
+                    cheb_enable=True, cheb_warmup=5, cheb_gamma=0.6666))
+    # two tets sharing a face, falling onto a non-fixed plane
+    v = np.array([[0, 0.02, 0], [1, 0.02, 0], [0, 1.02, 0], [0, 0.02, 1], [1, 1.02, 1]],
+                 dtype=np.float32)
+    t = np.array([[0, 1, 2, 3], [1, 4, 2, 3]], dtype=np.int32)
+    out["tet2_fall"] = dict(
+        verts=v, tets=t, sdfs=[plane(0.0, False)], gravity=(0.0, -9.8, 0.0), total_mass=1.0,
+        frames=6,
+        params=dict(young_modulus=1e5, poisson_ratio=0.45, frame_dt=1 / 60, dt=1 / 60,
+                    substeps_num=4, constraints_iter=15, block_neohookean_enable=True,
+                    cheb_enable=True, cheb_warmup=5, cheb_gamma=0.6666))
+    # 2^3 box resting on the plane (block, Chebyshev)
+    m = box_grid(2, origin=(0.0, 0.0, 0.0))
+    out["box2_contact"] = dict(
+        verts=m.verts, tets=m.tets, sdfs=[plane(0.0, True)], gravity=(0.0, -9.8, 0.0),
+        total_mass=1000.0, frames=5,
+        params=dict(young_modulus=1e5, poisson_ratio=0.45, frame_dt=1 / 60, dt=1 / 60,
+                    substeps_num=5, constraints_iter=30, block_neohookean_enable=True,
+                    cheb_enable=True, cheb_warmup=5, cheb_gamma=0.6666))
+    # jittered 4^3 box on a plane with a fixed sphere, split mode with the
+    # convergence monitor, without and with Chebyshev
+    m = box_grid(4, jitter=0.2, seed=3, origin=(0.0, 0.0, 0.0))
+    for name, cheb in (("box4_split_monitor", False), ("box4_split_cheb", True)):
+        out[name] = dict(
+            verts=m.verts, tets=m.tets, sdfs=[plane(0.0, True), sphere((0.5, 0.0, 0.5), 0.3, True)],
+            gravity=(0.0, -9.8, 0.0), total_mass=1000.0, frames=3,
+            params=dict(young_modulus=1e6, poisson_ratio=0.45, frame_dt=1 / 60, dt=1 / 60,
+                        substeps_num=3, constraints_iter=10, block_neohookean_enable=False,
+                        cheb_enable=cheb, cheb_warmup=5, cheb_gamma=0.6666,
+                        monitor_convergence=True, hydrostatic_relaxation=1.2,
+                        deviatoric_relaxation=1.2))
+    # the reference demo (cow), from frame 54 of its run
+    cow = read_tetgen(os.path.join(reference, "assets", "tetmesh", "cow.node"))
+    demo = dict(young_modulus=5e8, poisson_ratio=0.4999, scale=1.0, offset=(0.0, 1.0, 0.0),
+                frame_dt=1 / 60, dt=0.01, substeps_num=1, constraints_iter=200,
+                hydrostatic_relaxation=1.2, deviatoric_relaxation=1.2, cheb_enable=True,
+                cheb_warmup=5, cheb_gamma=0.666, block_neohookean_enable=True)
+    sdfs = [plane(-2.0, True), sphere((0.0, -1.0, 0.0), 0.3, True)]
+    warm = Oracle(cow.verts, cow.tets, sdfs, precision=32, threads=8, **demo)
+    warm.compute_lumped_mass(1.0)
+    warm.set_log(False)
+    warm.frame(54)
+    out["cow_demo_contact"] = dict(
+        verts=cow.verts, tets=cow.tets, x0=warm.x.astype(np.float32),
+        v0=warm.v.astype(np.float32), rho0=float(warm.rho_hat), sdfs=sdfs,
+        gravity=(0.0, -9.8, 0.0), total_mass=1.0, frames=6, params=demo)
+    return out
+
+
+def write(name, case):
+    x32, log32 = run_oracle(case, 32)
+    x64, log64 = run_oracle(case, 64)
+    arrays = dict(
+        verts=np.asarray(case["verts"], np.float32), tets=np.asarray(case["tets"], np.int32),
+        x0=np.asarray(case["x0"], np.float32) if case.get("x0") is not None else np.zeros((0, 3), np.float32),
+        v0=np.asarray(case["v0"], np.float32) if case.get("v0") is not None else np.zeros((0, 3), np.float32),
+        meta=np.array(json.dumps(dict(
+            sdfs=case["sdfs"], params=case["params"], gravity=list(case["gravity"]),
+            total_mass=case["total_mass"], frames=case["frames"], rho0=case.get("rho0")))),
+        x32=x32.astype(np.float32), x64=x64, log32=log32, log64=log64)
+    path = os.path.join(HERE, name + ".npz")
+    np.savez_compressed(path, **arrays)
+    print(f"{path}: {os.path.getsize(path) / 1024:.0f} KiB, {len(log32)} log rows", flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--reference", default="/root/reference")
+    args = ap.parse_args()
+    for name, case in cases(args.reference).items():
+        write(name, case)
+
+
+if __name__ == "__main__":
+    main()
