@@ -648,7 +648,17 @@ __global__ void __launch_bounds__(kPatchThreads)
       if (csr) {
         a = make_float4(0.f, 0.f, 0.f, 0.f);
         const int cb = vs0.x & 0xffff, ce = cb + ((vs0.x >> 16) & 0x7fff);
-        for (int c = cb; c < ce; ++c) {
+        int c = cb;
+        // four reads in flight, added in the same order as one at a time
+        for (; c + 4 <= ce; c += 4) {
+          const float4 v0 = s_sorted[c], v1 = s_sorted[c + 1], v2 = s_sorted[c + 2],
+                       v3 = s_sorted[c + 3];
+          a.x += v0.x; a.y += v0.y; a.z += v0.z; a.w += v0.w;
+          a.x += v1.x; a.y += v1.y; a.z += v1.z; a.w += v1.w;
+          a.x += v2.x; a.y += v2.y; a.z += v2.z; a.w += v2.w;
+          a.x += v3.x; a.y += v3.y; a.z += v3.z; a.w += v3.w;
+        }
+        for (; c < ce; ++c) {
           const float4 v = s_sorted[c];
           a.x += v.x;
           a.y += v.y;
