@@ -141,7 +141,7 @@ def committed_traffic(path: str = PMC_SUMMARY):
         with open(path) as fh:
             for line in fh:
                 if line.startswith("traffic_bytes_per_launch xpbd::k_patch_project:"):
-                    return int(float(line.split(":")[1]))
+                    return int(float(line.rsplit(":", 1)[1]))
     except OSError:
         pass
     return None
