@@ -707,7 +707,9 @@ __device__ __forceinline__ float4 stage_jacobi(const SceneDev& S, int g) {
 // Persistent patch residual, pipelined like k_patch_project: x_J (Chebyshev
 // on) or pos (off) and the rest positions of a patch are staged in LDS; the
 // sums (r, |C_H|, |C_D|) of each group of kPartialGroup patches form one
-// partial of the scene.  Each range starts at a group boundary.
+// partial of the scene: each lane adds its tets over the group's patches and
+// the workgroup reduces once per group, in a fixed order.  Each range starts
+// at a group boundary.
 __global__ void __launch_bounds__(kPatchThreads)
     k_patch_residual(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
                      const int2* __restrict__ ranges) {
@@ -739,10 +741,9 @@ __global__ void __launch_bounds__(kPatchThreads)
     h1 = items[rg.x + 1];
     if (tid < h1.n_verts) g1 = scenes[h1.scene].p_vert_ids[h1.vert_begin + tid];
   }
-  // running sums of the current patch group (thread 0); the range starts at a
-  // group boundary
-  __shared__ double grp[3];
-  if (tid == 0) grp[0] = grp[1] = grp[2] = 0.0;
+  // each lane's running sums over the current patch group, reduced over the
+  // workgroup at the group's last patch (the range starts at a group boundary)
+  float acc_r = 0.f, acc_h = 0.f, acc_d = 0.f;
   for (int i = rg.x; i < rg.y; ++i) {
     const SceneDev& S = scenes[h0.scene];
     if (tid < h0.n_verts) {
@@ -771,7 +772,6 @@ __global__ void __launch_bounds__(kPatchThreads)
       h2 = items[i + 2];
       if (tid < h2.n_verts) g2 = scenes[h2.scene].p_vert_ids[h2.vert_begin + tid];
     }
-    float r = 0.f, ah = 0.f, ad = 0.f;
     if (tid < h0.n_tets) {
       const int l[4] = {int(loc0.x & 0xffu), int((loc0.x >> 8) & 0xffu),
                         int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
@@ -779,22 +779,18 @@ __global__ void __launch_bounds__(kPatchThreads)
       float inv_det;
       rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm,
                  inv_det);
+      float r, ah, ad;
       residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, inv_det, lam0,
                    r, ah, ad);
+      acc_r += r;
+      acc_h += ah;
+      acc_d += ad;
     }
-    {
+    if ((h0.patch + 1) % kPartialGroup == 0 || h0.patch + 1 == S.n_patches) {  // uniform
       double tot[3];
-      block_sums(r, ah, ad, S.monitor != 0, tot);
-      if (tid == 0) {  // running sums of the patch group; stored at its last patch
-        grp[0] += tot[0];
-        grp[1] += tot[1];
-        grp[2] += tot[2];
-        if ((h0.patch + 1) % kPartialGroup == 0 || h0.patch + 1 == S.n_patches) {
-          const double g3[3] = {grp[0], grp[1], grp[2]};
-          store_partial(S, h0.patch / kPartialGroup, g3);
-          grp[0] = grp[1] = grp[2] = 0.0;
-        }
-      }
+      block_sums(acc_r, acc_h, acc_d, S.monitor != 0, tot);
+      if (tid == 0) store_partial(S, h0.patch / kPartialGroup, tot);
+      acc_r = acc_h = acc_d = 0.f;
     }
     __syncthreads();
     h0 = h1;

@@ -10,9 +10,9 @@ nu 0.4999) -- the position gate is relative to FP64 instead: the device's
 error against the f64 positions must not exceed twice the f32 oracle's.
 Likewise for the residual curve: where the f32 oracle's own r_hat departs
 from its f64 r_hat by more than 1 % on a gated iteration, the curve is
-round-off dominated and each gated iteration must satisfy
-|r_gpu - r_f64| <= 3 |r_f32 - r_f64| + 1 % r_f64 (as close to FP64 as the f32
-reference, up to a factor 3).
+round-off dominated; then, per substep, the device's median relative
+deviation from the f64 curve must be at most 3x the f32 oracle's plus 1 %, and
+its largest at most 10x the oracle's largest plus 1 %.
 """
 import numpy as np
 import pytest
@@ -54,17 +54,28 @@ def curve_is_resolved(case) -> bool:
 
 
 def check_curve_vs_fp64(solver, case):
+    """Round-off dominated curves: per substep, the median relative deviation of
+    the device's r_hat from f64 over the gated iterations must be at most 3x the
+    f32 oracle's plus 1 %, and no gated iteration may deviate by more than 10x
+    the f32 oracle's worst plus 1 % (no divergence)."""
     it = int(case["params"]["constraints_iter"])
     g = np.array([r["r_hat"] for r in solver.residual_history])
     assert len(g) == len(case["log64"])
     m = gated_rows(case["log64"], it)
-    r32, r64, rg = case["log32"][m, 3], case["log64"][m, 3], g[m]
-    band = 3.0 * np.abs(r32 - r64) + RES_TOL * r64
-    worst = float(np.max(np.abs(rg - r64) / band))
-    print(f"  residual curve (round-off dominated): worst |gpu - f64| / band = {worst:.3f} "
-          f"over {int(m.sum())} gated iterations", flush=True)
+    worst_med, worst_max = 0.0, 0.0
+    for s0 in range(0, len(g), it):
+        mm = m[s0:s0 + it]
+        if not mm.any():
+            continue
+        r64 = case["log64"][s0:s0 + it, 3][mm]
+        d32 = np.abs(case["log32"][s0:s0 + it, 3][mm] - r64) / r64
+        dg = np.abs(g[s0:s0 + it][mm] - r64) / r64
+        worst_med = max(worst_med, float(np.median(dg) / (3.0 * np.median(d32) + RES_TOL)))
+        worst_max = max(worst_max, float(dg.max() / (10.0 * d32.max() + RES_TOL)))
+    print(f"  residual curve (round-off dominated): median ratio {worst_med:.3f}, "
+          f"max ratio {worst_max:.3f} over {int(m.sum())} gated iterations", flush=True)
     assert m.sum() > 0
-    assert worst <= 1.0
+    assert worst_med <= 1.0 and worst_max <= 1.0
 
 
 @pytest.mark.parametrize("name", golden_cases.names())
