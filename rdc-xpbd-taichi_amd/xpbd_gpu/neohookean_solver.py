@@ -286,14 +286,19 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         recs = self._records(rows, per_iter)
         if self.monitor_convergence:
             self.convergence_history.extend(recs)
-            by_sub = {}
+            # one callback per substep of every time chunk, in order, as the
+            # reference calls it after each substep (neohookean_solver.py:576-578)
+            runs = []
             for rec in recs:
-                by_sub.setdefault(rec["substep"], []).append(rec)
-            for sub in sorted(by_sub):
+                if rec["iter"] == 0 or not runs:
+                    runs.append([])
+                runs[-1].append(rec)
+            for run in runs:
+                sub = run[0]["substep"]
                 self.current_substep = sub
-                self.current_substep_convergence = by_sub[sub]
+                self.current_substep_convergence = run
                 if self.convergence_callback:
-                    self.convergence_callback(self.current_frame, sub, by_sub[sub])
+                    self.convergence_callback(self.current_frame, sub, run)
         self.current_substep = substep.value
         self.time = t.value
         self.current_frame = frame.value
