@@ -4,7 +4,7 @@
  * The reference exposes this path as a Python class, not an FFI:
  *   xpbd_gpu/neohookean_solver.py:15-587  XPBDNeoHookeanSolver
  *   xpbd_gpu/xpbd_base.py:11-204          XPBDSolverBase
- *   xpbd_gpu/sdf.py:59-95                 PlaneSdf, SphereSdf
+ *   xpbd_gpu/sdf.py:29-124                PlaneSdf, SphereSdf, CubeSdf, HangSdfModel
  * Each entry point below names the reference method it replaces.  The Python
  * drop-in package (xpbd_gpu) binds these with ctypes.
  *
@@ -49,12 +49,20 @@ extern "C" {
 /* kind of an SDF collider (xpbd_gpu/sdf.py) */
 #define XPBD_SDF_PLANE 0  /* PlaneSdf(point, dir, fixed), sdf.py:59-74 */
 #define XPBD_SDF_SPHERE 1 /* SphereSdf(center, radius, fixed), sdf.py:77-95 */
+#define XPBD_SDF_CUBE 2   /* CubeSdf(center, size, fixed), sdf.py:98-124 */
+#define XPBD_SDF_HANG 3   /* HangSdfModel(pos), sdf.py:29-56: union of spheres, always fixed */
+#define XPBD_MAX_HANG_POINTS 4096
 
 typedef struct xpbd_sdf {
   int32_t kind;
-  int32_t fixed; /* pins vertices that start inside; always projects (sdf.py:24) */
-  double p[4];   /* plane: point xyz | sphere: center xyz, radius */
-  double dir[3]; /* plane: normal (normalized by the library, sdf.py:66) */
+  int32_t fixed;   /* pins vertices that start inside; always projects (sdf.py:24) */
+  double p[4];     /* plane: point xyz | sphere: center xyz, radius | cube: center xyz |
+                      hang: p[3] = sphere radius (0.013 in the reference, sdf.py:34) */
+  double dir[3];   /* plane: normal (normalized by the library, sdf.py:66) | cube: edge
+                      lengths (size; the half size is size * 0.5 in f32, sdf.py:104) */
+  const float* points; /* hang: n_points x 3 sphere centres, copied by the library */
+  int32_t n_points;    /* hang: 1 .. XPBD_MAX_HANG_POINTS */
+  int32_t pad;
 } xpbd_sdf;
 
 /* Constructor arguments of XPBDNeoHookeanSolver (neohookean_solver.py:18-38)
@@ -96,7 +104,7 @@ typedef struct xpbd_solver xpbd_solver;
 
 /* Version of this ABI and the sizes of its structs as compiled, so that a
  * binding can check its own struct layout. */
-#define XPBD_ABI_VERSION 2
+#define XPBD_ABI_VERSION 3
 int xpbd_abi_info(int32_t* version, size_t* desc_size, size_t* sdf_size);
 
 /* Fills *d with the reference defaults (everything but the mesh). */

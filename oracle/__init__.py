@@ -74,6 +74,10 @@ def _load():
     lib.oracle_rho_hat.restype = ctypes.c_double
     lib.oracle_set_rho_hat.argtypes = [vp, ctypes.c_double]
     lib.oracle_set_rho_hat.restype = None
+    lib.oracle_set_sdf_points.argtypes = [vp, ctypes.c_int, dp, ctypes.c_int]
+    lib.oracle_set_sdf_points.restype = None
+    lib.oracle_sdf_eval.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+    lib.oracle_sdf_eval.restype = ctypes.c_int
     lib.oracle_set_log.argtypes = [vp, ctypes.c_int]
     lib.oracle_log_size.argtypes = [vp]
     lib.oracle_log_size.restype = ctypes.c_int
@@ -91,7 +95,8 @@ def _dptr(a: np.ndarray):
 
 
 def sdf_arrays(sdfs):
-    """sdfs: list of dicts {kind: 'plane'|'sphere', point/dir | center/radius, fixed}."""
+    """sdfs: list of dicts {kind: 'plane'|'sphere'|'cube'|'hang', point/dir |
+    center/radius | center/size | points/radius, fixed}."""
     n = len(sdfs)
     kind = np.zeros(n, dtype=np.int32)
     fixed = np.zeros(n, dtype=np.int32)
@@ -108,6 +113,14 @@ def sdf_arrays(sdfs):
             kind[i] = 1
             p[i, :3] = s["center"]
             p[i, 3] = s["radius"]
+        elif s["kind"] == "cube":
+            kind[i] = 2
+            p[i, :3] = s["center"]
+            d[i] = s["size"]
+        elif s["kind"] == "hang":
+            kind[i] = 3
+            fixed[i] = 1
+            p[i, 3] = s.get("radius", 0.013)
         else:
             raise ValueError(s["kind"])
     return kind, fixed, p, d
@@ -141,6 +154,10 @@ class Oracle:
             kind.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
             fixed.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dptr(p), _dptr(d), threads)
         self._lib = lib
+        for i, s in enumerate(sdfs):
+            if s["kind"] == "hang":
+                pts = np.ascontiguousarray(s["points"], dtype=np.float64).reshape(-1, 3)
+                lib.oracle_set_sdf_points(self._h, i, _dptr(pts), pts.shape[0])
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -201,6 +218,14 @@ class Oracle:
 
     def compute_lumped_mass(self, total_mass: float):
         self._lib.oracle_compute_lumped_mass(self._h, float(total_mass))
+
+    def sdf_eval(self, i: int, x, v=(0.0, 0.0, 0.0)):
+        """(dist, normal, inside) of collider i at x for a vertex moving with v."""
+        xa = np.ascontiguousarray(x, dtype=np.float64).reshape(3)
+        va = np.ascontiguousarray(v, dtype=np.float64).reshape(3)
+        out = np.zeros(4, dtype=np.float64)
+        inside = self._lib.oracle_sdf_eval(self._h, int(i), _dptr(xa), _dptr(va), _dptr(out))
+        return out[0], out[1:].copy(), bool(inside)
 
     @property
     def rho_hat(self) -> float:

@@ -183,38 +183,82 @@ inline int solve2x2_pivot_lu(R a00, R a01, R a10, R a11, R b0, R b1, R eps, R* x
 }
 
 struct SdfDesc {
-  int kind;        // 0 plane, 1 sphere
+  int kind;        // 0 plane, 1 sphere, 2 cube, 3 hang
   int fixed;
-  double p[4];     // plane: point xyz | sphere: center xyz, radius
-  double dir[3];   // plane: unit normal (normalized in double at construction, sdf.py:66)
+  double p[4];     // plane: point xyz | sphere: center xyz, radius | cube: center | hang: p[3] radius
+  double dir[3];   // plane: unit normal (normalized in double at construction, sdf.py:66) |
+                   // cube: edge lengths
 };
 
+// One collider as the reference's f32 fields hold it (sdf.py:6-124).  Taichi's
+// norm(eps) is sqrt(|q|^2 + eps), normalized(eps) is q / (|q| + eps).
 template <typename R>
 struct Sdf {
   int kind, fixed;
   R p[3], r, d[3];
+  std::vector<R> pts;  // hang: sphere centres, xyz per sphere
   explicit Sdf(const SdfDesc& s) : kind(s.kind), fixed(s.fixed) {
     for (int i = 0; i < 3; ++i) {
       p[i] = R(s.p[i]);
-      d[i] = R(s.dir[i]);
+      d[i] = kind == 2 ? R(s.dir[i] * 0.5) : R(s.dir[i]);  // cube: half_size = size * 0.5
     }
     r = R(s.p[3]);
+    if (kind == 3) fixed = 1;  // HangSdfModel(fixed=True), sdf.py:31
   }
-  // PlaneSdf.dist (sdf.py:69-70) / SphereSdf.dist (sdf.py:87-88)
   R dist(const R* x) const {
+    if (kind == 3) {  // HangSdfModel.dist (sdf.py:38-43)
+      R dist = R(1e5);
+      for (size_t i = 0; i + 2 < pts.size(); i += 3) {
+        const R q[3] = {x[0] - pts[i], x[1] - pts[i + 1], x[2] - pts[i + 2]};
+        const R di = std::sqrt(dot3(q, q) + R(kEpsilon)) - r;
+        dist = std::min(di, dist);
+      }
+      return dist;
+    }
     R q[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
-    if (kind == 0) return dot3(q, d);
-    return std::sqrt(dot3(q, q)) - r;
+    if (kind == 0) return dot3(q, d);                   // PlaneSdf.dist (sdf.py:69-70)
+    if (kind == 1) return std::sqrt(dot3(q, q)) - r;    // SphereSdf.dist (sdf.py:87-88)
+    R e[3], o[3];                                        // CubeSdf.dist (sdf.py:107-110)
+    for (int k = 0; k < 3; ++k) {
+      e[k] = std::fabs(q[k]) - d[k];
+      o[k] = std::max(e[k], R(0));
+    }
+    return std::sqrt(dot3(o, o)) + std::min(std::max(e[0], std::max(e[1], e[2])), R(0));
   }
-  // PlaneSdf.normal (sdf.py:72-74) / SphereSdf.normal = normalized(EPSILON) (sdf.py:90-92)
   void normal(const R* x, R* n) const {
-    if (kind == 0) {
+    n[0] = n[1] = n[2] = R(0);
+    if (kind == 0) {  // PlaneSdf.normal (sdf.py:72-74)
       for (int i = 0; i < 3; ++i) n[i] = d[i];
       return;
     }
+    if (kind == 3) {  // HangSdfModel.normal (sdf.py:45-53): towards sphere 0
+      R dist = R(1e5);
+      bool hit = false;
+      for (size_t i = 0; i + 2 < pts.size(); i += 3) {
+        const R q[3] = {x[0] - pts[i], x[1] - pts[i + 1], x[2] - pts[i + 2]};
+        const R di = std::sqrt(dot3(q, q) + R(kEpsilon)) - r;
+        if (di < dist) {
+          dist = di;
+          hit = true;
+        }
+      }
+      if (hit) {
+        const R q[3] = {x[0] - pts[0], x[1] - pts[1], x[2] - pts[2]};
+        const R inv_len = R(1) / (std::sqrt(dot3(q, q)) + R(kEpsilon));
+        for (int i = 0; i < 3; ++i) n[i] = inv_len * q[i];
+      }
+      return;
+    }
     R q[3] = {x[0] - p[0], x[1] - p[1], x[2] - p[2]};
-    const R inv_len = R(1) / (std::sqrt(dot3(q, q)) + R(kEpsilon));
-    for (int i = 0; i < 3; ++i) n[i] = inv_len * q[i];
+    if (kind == 1) {  // SphereSdf.normal = normalized(EPSILON) (sdf.py:90-92)
+      const R inv_len = R(1) / (std::sqrt(dot3(q, q)) + R(kEpsilon));
+      for (int i = 0; i < 3; ++i) n[i] = inv_len * q[i];
+      return;
+    }
+    int axis = 0;  // CubeSdf.normal (sdf.py:112-124)
+    if (std::fabs(q[1]) > std::fabs(q[0])) axis = 1;
+    if (std::fabs(q[2]) > std::fabs(q[axis])) axis = 2;
+    n[axis] = q[axis] > R(0) ? R(1) : R(-1);
   }
   // SdfModel.check (sdf.py:13-27): returns inside, fills n.  The obstacle's
   // own velocity is always zero in the reference (never written).
@@ -323,6 +367,20 @@ class Solver {
   }
   void set_total_mass(double tm) { total_mass_ = tm; }
   double rho_hat() const { return rho_hat_; }
+  int sdf_eval(int i, const double* x, const double* v, double* out) const {
+    if (i < 0 || i >= int(sdfs_.size())) return -1;
+    const R xr[3] = {R(x[0]), R(x[1]), R(x[2])}, vr[3] = {R(v[0]), R(v[1]), R(v[2])};
+    R n[3];
+    out[0] = double(sdfs_[i].dist(xr));
+    sdfs_[i].normal(xr, n);
+    for (int k = 0; k < 3; ++k) out[1 + k] = double(n[k]);
+    return sdfs_[i].check(xr, vr, n) ? 1 : 0;
+  }
+  void set_sdf_points(int i, const double* xyz, int n) {
+    if (i < 0 || i >= int(sdfs_.size())) return;
+    sdfs_[i].pts.assign(3 * size_t(n), R(0));
+    for (size_t k = 0; k < 3 * size_t(n); ++k) sdfs_[i].pts[k] = R(float(xyz[k]));
+  }
   void set_rho_hat(double v) { rho_hat_ = v; }
   double time() const { return time_; }
   int frame() const { return frame_; }
@@ -794,7 +852,8 @@ Params params_from(const double* p) {
 extern "C" {
 
 // params: 22 doubles, in the order read by params_from().
-// sdf_kind[n] (0 plane, 1 sphere), sdf_fixed[n], sdf_p[4n], sdf_dir[3n].
+// sdf_kind[n] (0 plane, 1 sphere, 2 cube, 3 hang), sdf_fixed[n], sdf_p[4n], sdf_dir[3n];
+// a hang collider's centres follow with oracle_set_sdf_points.
 void* oracle_create(int precision, int nv, int nt, const float* rest, const int* tets,
                     const double* params, int n_sdf, const int* sdf_kind, const int* sdf_fixed,
                     const double* sdf_p, const double* sdf_dir, int threads) {
@@ -866,6 +925,15 @@ void oracle_compute_lumped_mass(void* hp, double total_mass) {
 }
 double oracle_rho_hat(void* hp) {
   return dispatch(static_cast<Handle*>(hp), [](auto& s) { return s.rho_hat(); });
+}
+// dist (out[0]) and normal (out[1..3]) of collider i at x, and whether a
+// vertex with velocity v counts as inside (returned)
+int oracle_sdf_eval(void* hp, int i, const double* x, const double* v, double* out) {
+  return dispatch(static_cast<Handle*>(hp), [&](auto& s) { return s.sdf_eval(i, x, v, out); });
+}
+// sphere centres (xyz per sphere) of hang collider i, as f32 values
+void oracle_set_sdf_points(void* hp, int i, const double* xyz, int n) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_sdf_points(i, xyz, n); return 0; });
 }
 // restores a saved rho_hat (a run continued from a stored state)
 void oracle_set_rho_hat(void* hp, double v) {

@@ -40,12 +40,13 @@ template <class T>
 using gptr = XPBD_GLOBAL_AS T*;
 
 struct SdfDev {
-  int kind;   // XPBD_SDF_PLANE / XPBD_SDF_SPHERE
+  int kind;   // XPBD_SDF_PLANE / SPHERE / CUBE / HANG
   int fixed;
-  float p[3];
-  float r;
-  float d[3];
-  float pad;
+  float p[3];  // plane point | sphere or cube centre
+  float r;     // sphere radius | hang sphere radius
+  float d[3];  // plane unit normal | cube half size
+  int n_pts;   // hang: number of spheres
+  gptr<const float4> pts;  // hang: sphere centres (xyz)
 };
 
 // Per-scene scalar state that persists across launches (rho_hat persists over
@@ -284,31 +285,97 @@ __device__ __forceinline__ void solve2x2_with_fallback(float A11, float A12, flo
   }
 }
 
-// SdfModel.check + dist + normal for PlaneSdf / SphereSdf (sdf.py:13-27, 59-95).
-__device__ __forceinline__ float sdf_dist(const SdfDev& s, float3 x) {
-  const float3 q = x - make_float3(s.p[0], s.p[1], s.p[2]);
-  if (s.kind == XPBD_SDF_PLANE) return dot(q, make_float3(s.d[0], s.d[1], s.d[2]));
-  return sqrtf(dot(q, q)) - s.r;
+// SdfModel.check + dist + normal (sdf.py:13-124) in scalar f32 arithmetic,
+// shared by the kernels and the host's fixed-vertex pinning.  Taichi's
+// norm(eps) is sqrt(|q|^2 + eps) and normalized(eps) is q / (|q| + eps).
+__host__ __device__ inline float sdf_dist(const SdfDev& s, const float x[3]) {
+  if (s.kind == XPBD_SDF_HANG) {  // HangSdfModel.dist (sdf.py:38-43)
+    float dist = 1e5f;
+    for (int i = 0; i < s.n_pts; ++i) {
+      const float4 c = s.pts[i];
+      const float q0 = x[0] - c.x, q1 = x[1] - c.y, q2 = x[2] - c.z;
+      const float di = sqrtf(q0 * q0 + q1 * q1 + q2 * q2 + kEps) - s.r;
+      dist = di < dist ? di : dist;
+    }
+    return dist;
+  }
+  const float q0 = x[0] - s.p[0], q1 = x[1] - s.p[1], q2 = x[2] - s.p[2];
+  if (s.kind == XPBD_SDF_PLANE) return q0 * s.d[0] + q1 * s.d[1] + q2 * s.d[2];
+  if (s.kind == XPBD_SDF_SPHERE) return sqrtf(q0 * q0 + q1 * q1 + q2 * q2) - s.r;
+  // CubeSdf.dist (sdf.py:107-110): |max(d, 0)| + min(max(d.x, d.y, d.z), 0)
+  const float e0 = fabsf(q0) - s.d[0], e1 = fabsf(q1) - s.d[1], e2 = fabsf(q2) - s.d[2];
+  const float o0 = fmaxf(e0, 0.f), o1 = fmaxf(e1, 0.f), o2 = fmaxf(e2, 0.f);
+  return sqrtf(o0 * o0 + o1 * o1 + o2 * o2) + fminf(fmaxf(e0, fmaxf(e1, e2)), 0.f);
 }
 
-__device__ __forceinline__ float3 sdf_normal(const SdfDev& s, float3 x) {
-  if (s.kind == XPBD_SDF_PLANE) return make_float3(s.d[0], s.d[1], s.d[2]);
-  const float3 q = x - make_float3(s.p[0], s.p[1], s.p[2]);
-  const float inv_len = 1.f / (sqrtf(dot(q, q)) + kEps);
-  return inv_len * q;
+__host__ __device__ inline void sdf_normal(const SdfDev& s, const float x[3], float n[3]) {
+  if (s.kind == XPBD_SDF_PLANE) {
+    n[0] = s.d[0];
+    n[1] = s.d[1];
+    n[2] = s.d[2];
+    return;
+  }
+  if (s.kind == XPBD_SDF_HANG) {
+    // HangSdfModel.normal (sdf.py:45-53): the nearest sphere is searched, but
+    // the normal is taken from sphere 0 (the reference's behaviour, kept)
+    n[0] = n[1] = n[2] = 0.f;
+    if (s.n_pts <= 0) return;
+    float dist = 1e5f;
+    bool hit = false;
+    for (int i = 0; i < s.n_pts; ++i) {
+      const float4 c = s.pts[i];
+      const float q0 = x[0] - c.x, q1 = x[1] - c.y, q2 = x[2] - c.z;
+      const float di = sqrtf(q0 * q0 + q1 * q1 + q2 * q2 + kEps) - s.r;
+      if (di < dist) {
+        dist = di;
+        hit = true;
+      }
+    }
+    if (hit) {
+      const float4 c = s.pts[0];
+      const float q0 = x[0] - c.x, q1 = x[1] - c.y, q2 = x[2] - c.z;
+      const float inv = 1.f / (sqrtf(q0 * q0 + q1 * q1 + q2 * q2) + kEps);
+      n[0] = inv * q0;
+      n[1] = inv * q1;
+      n[2] = inv * q2;
+    }
+    return;
+  }
+  const float q[3] = {x[0] - s.p[0], x[1] - s.p[1], x[2] - s.p[2]};
+  if (s.kind == XPBD_SDF_SPHERE) {
+    const float inv = 1.f / (sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) + kEps);
+    n[0] = inv * q[0];
+    n[1] = inv * q[1];
+    n[2] = inv * q[2];
+    return;
+  }
+  // CubeSdf.normal (sdf.py:112-124): the axis of the largest |x - centre|
+  int axis = 0;
+  if (fabsf(q[1]) > fabsf(q[0])) axis = 1;
+  if (fabsf(q[2]) > fabsf(q[axis])) axis = 2;
+  n[0] = n[1] = n[2] = 0.f;
+  n[axis] = q[axis] > 0.f ? 1.f : -1.f;
+}
+
+// SdfModel.check (sdf.py:13-27) with the obstacle's velocity, which the
+// reference never sets, at zero: inside when phi < 0 and (n . (0 - v) > 0 or
+// fixed).
+__host__ __device__ inline bool sdf_inside(const SdfDev& s, const float x[3], const float v[3],
+                                           float n[3]) {
+  n[0] = n[1] = n[2] = 0.f;
+  if (!(sdf_dist(s, x) < 0.f)) return false;
+  sdf_normal(s, x, n);
+  const float dotnv = n[0] * (0.f - v[0]) + n[1] * (0.f - v[1]) + n[2] * (0.f - v[2]);
+  return dotnv > 0.f || s.fixed;
 }
 
 // One collider projection of one vertex (xpbd_base.py:162-174).
 __device__ __forceinline__ float3 sdf_project(const SdfDev& s, float3 x, float3 v) {
-  const float phi = sdf_dist(s, x);
-  if (phi < 0.f) {
-    const float3 n = sdf_normal(s, x);
-    const float3 dv = make_float3(0.f - v.x, 0.f - v.y, 0.f - v.z);
-    const float dotnv = dot(n, dv);
-    if (dotnv > 0.f || s.fixed) {
-      const float depth = -sdf_dist(s, x);
-      if (depth > 0.f) x = x + n * depth;
-    }
+  const float xa[3] = {x.x, x.y, x.z}, va[3] = {v.x, v.y, v.z};
+  float n[3];
+  if (sdf_inside(s, xa, va, n)) {
+    const float depth = -sdf_dist(s, xa);
+    if (depth > 0.f) x = x + make_float3(n[0], n[1], n[2]) * depth;
   }
   return x;
 }

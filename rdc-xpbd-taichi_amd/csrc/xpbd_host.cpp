@@ -125,48 +125,42 @@ void inv3_f32(const float m[3][3], float out[3][3]) {
     }
 }
 
-SdfDev to_dev(const xpbd_sdf& s) {
+// Device form of a collider (the parameters as the reference's f32 fields
+// hold them); a hang collider's sphere centres at `pts`.
+SdfDev to_dev(const xpbd_sdf& s, const float4* pts) {
   SdfDev d;
   std::memset(&d, 0, sizeof(d));
   d.kind = s.kind;
   d.fixed = s.fixed;
   for (int k = 0; k < 3; ++k) d.p[k] = float(s.p[k]);
   d.r = float(s.p[3]);
-  // PlaneSdf normalizes dir at construction in Python (sdf.py:66).
-  const double n = std::sqrt(s.dir[0] * s.dir[0] + s.dir[1] * s.dir[1] + s.dir[2] * s.dir[2]);
-  for (int k = 0; k < 3; ++k) d.d[k] = n > 0 ? float(s.dir[k] / n) : 0.f;
+  if (s.kind == XPBD_SDF_PLANE) {
+    // PlaneSdf normalizes dir at construction in Python (sdf.py:66).
+    const double n = std::sqrt(s.dir[0] * s.dir[0] + s.dir[1] * s.dir[1] + s.dir[2] * s.dir[2]);
+    for (int k = 0; k < 3; ++k) d.d[k] = n > 0 ? float(s.dir[k] / n) : 0.f;
+  } else if (s.kind == XPBD_SDF_CUBE) {
+    for (int k = 0; k < 3; ++k) d.d[k] = float(s.dir[k] * 0.5);  // half_size (sdf.py:104)
+  } else if (s.kind == XPBD_SDF_HANG) {
+    d.fixed = 1;  // HangSdfModel is always fixed (sdf.py:31)
+    d.n_pts = s.n_points;
+    d.pts = pts;
+  }
   return d;
-}
-
-// Host copy of SdfModel.check (sdf.py:13-27) used by the fixed-vertex pinning
-// of compute_lumped_mass (xpbd_base.py:124-131).
-bool sdf_inside_host(const SdfDev& s, const float* x, const float* v) {
-  const float q[3] = {x[0] - s.p[0], x[1] - s.p[1], x[2] - s.p[2]};
-  float phi, n[3];
-  if (s.kind == XPBD_SDF_PLANE) {
-    phi = q[0] * s.d[0] + q[1] * s.d[1] + q[2] * s.d[2];
-  } else {
-    phi = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) - s.r;
-  }
-  if (!(phi < 0.f)) return false;
-  if (s.kind == XPBD_SDF_PLANE) {
-    for (int k = 0; k < 3; ++k) n[k] = s.d[k];
-  } else {
-    const float inv_len = 1.f / (std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2]) + 1e-12f);
-    for (int k = 0; k < 3; ++k) n[k] = inv_len * q[k];
-  }
-  const float dv[3] = {0.f - v[0], 0.f - v[1], 0.f - v[2]};
-  const float dotnv = n[0] * dv[0] + n[1] * dv[1] + n[2] * dv[2];
-  return dotnv > 0.f || s.fixed;
 }
 
 int validate_sdfs(const xpbd_sdf* sdfs, int n) {
   if (n < 1) return set_error(XPBD_EINVAL, "At least one SDF object must be provided.");
   if (n > XPBD_MAX_SDF) return set_error(XPBD_EINVAL, "at most %d SDFs are supported", XPBD_MAX_SDF);
   if (!sdfs) return set_error(XPBD_EINVAL, "sdfs is NULL");
-  for (int i = 0; i < n; ++i)
-    if (sdfs[i].kind != XPBD_SDF_PLANE && sdfs[i].kind != XPBD_SDF_SPHERE)
-      return set_error(XPBD_EINVAL, "sdf %d: unknown kind %d", i, sdfs[i].kind);
+  for (int i = 0; i < n; ++i) {
+    const int k = sdfs[i].kind;
+    if (k != XPBD_SDF_PLANE && k != XPBD_SDF_SPHERE && k != XPBD_SDF_CUBE && k != XPBD_SDF_HANG)
+      return set_error(XPBD_EINVAL, "sdf %d: unknown kind %d", i, k);
+    if (k == XPBD_SDF_HANG &&
+        (!sdfs[i].points || sdfs[i].n_points < 1 || sdfs[i].n_points > XPBD_MAX_HANG_POINTS))
+      return set_error(XPBD_EINVAL, "sdf %d: a hang collider needs 1..%d points", i,
+                       XPBD_MAX_HANG_POINTS);
+  }
   return XPBD_OK;
 }
 
@@ -340,7 +334,9 @@ struct xpbd_solver {
   int device = 0;
   hipStream_t stream = nullptr;
   xpbd_desc prm{};                 // scalar parameters (mesh pointers cleared)
-  std::vector<xpbd_sdf> sdfs;
+  std::vector<xpbd_sdf> sdfs;                    // points copied into sdf_pts_*
+  std::vector<std::vector<float4>> sdf_pts_host;  // hang colliders' sphere centres
+  DevBuf<float4> sdf_pts_dev[XPBD_MAX_SDF];
   int nv = 0, nt = 0;
   int bs_t = 256, bs_v = 256;
   std::vector<int> v_new2old, v_old2new, t_new2old;
@@ -463,7 +459,7 @@ void fill_scene_params(xpbd_solver* s, SceneDev& S, double dt, double damping) {
   S.warmup = d.cheb_warmup;
   S.iters = d.constraints_iter;
   S.n_sdf = int(s->sdfs.size());
-  for (int i = 0; i < S.n_sdf; ++i) S.sdf[i] = to_dev(s->sdfs[i]);
+  for (int i = 0; i < S.n_sdf; ++i) S.sdf[i] = to_dev(s->sdfs[i], s->sdf_pts_dev[i].p);
 }
 
 int build_scene(xpbd_solver* s) {
@@ -637,12 +633,38 @@ void lumped_mass_host(xpbd_solver* s, const std::vector<float>& x, const std::ve
     s->m_host[i] *= mass_scale;
     s->inv_m_host[i] = s->m_host[i] > 0.f ? 1.f / s->m_host[i] : 0.f;
   }
-  for (const auto& sd : s->sdfs) {
-    const SdfDev d = to_dev(sd);
+  for (size_t j = 0; j < s->sdfs.size(); ++j) {
+    const SdfDev d = to_dev(s->sdfs[j], s->sdf_pts_host[j].data());
+    float n[3];
     for (int i = 0; i < s->nv; ++i)
-      if (s->inv_m_host[i] > 0.f && d.fixed && sdf_inside_host(d, &x[3 * size_t(i)], &v[3 * size_t(i)]))
+      if (s->inv_m_host[i] > 0.f && d.fixed &&
+          xpbd::sdf_inside(d, &x[3 * size_t(i)], &v[3 * size_t(i)], n))
         s->inv_m_host[i] = 0.f;
   }
+}
+
+// Keeps copies of the colliders; a hang collider's points are copied to the
+// host and to a device buffer of this solver.
+int store_sdfs(xpbd_solver* s, const xpbd_sdf* sdfs, int n) {
+  TRY(validate_sdfs(sdfs, n));
+  s->sdfs.assign(sdfs, sdfs + n);
+  s->sdf_pts_host.assign(n, {});
+  for (int i = 0; i < n; ++i) {
+    xpbd_sdf& c = s->sdfs[i];
+    if (c.kind == XPBD_SDF_HANG) {
+      std::vector<float4>& h = s->sdf_pts_host[i];
+      h.resize(c.n_points);
+      for (int k = 0; k < c.n_points; ++k)
+        h[k] = make_float4(c.points[3 * k], c.points[3 * k + 1], c.points[3 * k + 2], 0.f);
+      TRY(s->sdf_pts_dev[i].upload(h.data(), h.size()));
+    } else {
+      s->sdf_pts_dev[i].release();
+      c.n_points = 0;
+    }
+    c.points = nullptr;  // the caller's buffer is not kept
+  }
+  s->prm.n_sdf = n;
+  return XPBD_OK;
 }
 
 int download_vec4_xyz(xpbd_solver* s, const DevBuf<float4>& b, std::vector<float>& out,
@@ -899,7 +921,7 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
   s->prm.rest_positions = nullptr;
   s->prm.tets = nullptr;
   s->prm.sdfs = nullptr;
-  s->sdfs.assign(d->sdfs, d->sdfs + d->n_sdf);
+  TRY(store_sdfs(s.get(), d->sdfs, d->n_sdf));
   s->nv = nv;
   s->nt = nt;
   s->bs_t = d->block_size;
@@ -1158,9 +1180,7 @@ int xpbd_update_params(xpbd_solver* s, const xpbd_desc* d) {
 
 int xpbd_set_sdfs(xpbd_solver* s, const xpbd_sdf* sdfs, int32_t n) {
   CHECK_HANDLE(s);
-  TRY(validate_sdfs(sdfs, n));
-  s->sdfs.assign(sdfs, sdfs + n);
-  s->prm.n_sdf = n;
+  TRY(store_sdfs(s, sdfs, n));
   s->table_valid = false;
   return sync_table(s, double(s->host_scene.dt), s->prm.damping);
 }

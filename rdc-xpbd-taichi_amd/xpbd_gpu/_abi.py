@@ -22,12 +22,15 @@ XPBD_ENOMEM = -3
 XPBD_ESTATE = -4
 XPBD_ENUMERIC = -5
 XPBD_MAX_SDF = 8
-XPBD_ABI_VERSION = 2
+XPBD_ABI_VERSION = 3
 
 ACCUM_ATOMIC = 0
 ACCUM_CSR = 1
 SDF_PLANE = 0
 SDF_SPHERE = 1
+SDF_CUBE = 2
+SDF_HANG = 3
+MAX_HANG_POINTS = 4096
 PATH_KERNELS = 0
 PATH_GRAPH = 1
 LAYOUT_FLAT = 0
@@ -57,6 +60,9 @@ class XpbdSdf(ctypes.Structure):
         ("fixed", ctypes.c_int32),
         ("p", ctypes.c_double * 4),
         ("dir", ctypes.c_double * 3),
+        ("points", ctypes.POINTER(ctypes.c_float)),
+        ("n_points", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 

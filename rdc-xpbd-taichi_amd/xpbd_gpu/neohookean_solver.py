@@ -196,8 +196,7 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
 
     def _param_key(self):
         g = tuple(float(x) for x in np.asarray(self.gravity[None], dtype=np.float64))
-        sd = tuple((s.kind, int(bool(s.fixed[None])), tuple(s.to_abi().p), tuple(s.to_abi().dir))
-                   for s in self.sdfs)
+        sd = tuple(s.key() for s in self.sdfs)
         return tuple(getattr(self, a) for a in _PARAM_ATTRS) + (g, sd)
 
     def _sync_params(self):

@@ -43,6 +43,15 @@ def sphere(c, r, fixed=True):
     return dict(kind="sphere", center=tuple(c), radius=r, fixed=fixed)
 
 
+def cube(c, size, fixed=False):
+    return dict(kind="cube", center=tuple(c), size=tuple(size), fixed=fixed)
+
+
+def hang(points, radius=0.013):
+    return dict(kind="hang", points=[list(map(float, p)) for p in points], radius=radius,
+                fixed=True)
+
+
 def base_params(**kw):
     p = dict(frame_dt=1.0 / 60.0, dt=1.0 / 60.0, block_neohookean_enable=True, cheb_enable=True,
              cheb_warmup=5, cheb_gamma=0.6666)
@@ -142,13 +151,19 @@ def batched_scenes(n_scenes: int = 256, seed: int = 0, n_range=(6, 40), cell: fl
 
 def make_solver(scene: Scene, **opts):
     from .neohookean_solver import XPBDNeoHookeanSolver
-    from .sdf import PlaneSdf, SphereSdf
+    from .sdf import CubeSdf, HangSdfModel, PlaneSdf, SphereSdf
     sd = []
     for s in scene.sdfs:
         if s["kind"] == "plane":
             sd.append(PlaneSdf(point=s["point"], dir=s["dir"], fixed=s["fixed"]))
-        else:
+        elif s["kind"] == "sphere":
             sd.append(SphereSdf(center=s["center"], radius=s["radius"], fixed=s["fixed"]))
+        elif s["kind"] == "cube":
+            sd.append(CubeSdf(center=s["center"], size=s["size"], fixed=s["fixed"]))
+        else:
+            h = HangSdfModel(np.asarray(s["points"], dtype=np.float32))
+            h.sphere_radius = s.get("radius", 0.013)
+            sd.append(h)
     p = dict(scene.params)
     solver = XPBDNeoHookeanSolver((scene.mesh.verts, scene.mesh.tets), sd, **p, **opts)
     solver.gravity[None] = scene.gravity

@@ -331,3 +331,24 @@ def test_single_tet_free_fall_matches_closed_form():
             x = x + v * dt
             v = v * np.exp(-dt * 5.0)
     np.testing.assert_allclose(s.mesh.verts.x.to_numpy(), x, atol=2e-6)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_cube_and_hang_colliders(layout):
+    """A 4^3 box resting on a non-fixed CubeSdf table and hung from two top
+    corners by a HangSdfModel (which pins the vertices it holds at set-up)."""
+    from xpbd_gpu.scenes import Scene, base_params, cube, hang, plane
+    _print_case("cube_and_hang", layout=layout)
+    m = box_grid(4, origin=(0.0, 0.3, 0.0))
+    sdfs = [cube((0.5, 0.1, 0.5), (2.0, 0.4, 2.0), fixed=False),
+            hang([[0.0, 1.3, 0.0], [1.0, 1.3, 1.0]]), plane(-1.0)]
+    scene = Scene("cube_hang", m, sdfs,
+                  base_params(young_modulus=2e5, poisson_ratio=0.4, substeps_num=4,
+                              constraints_iter=20),
+                  total_mass=1000.0, frames=5)
+    solver, orc = run_pair(scene, layout=layout)
+    check_positions(scene, solver, orc)
+    check_residual_curve(solver, orc)
+    x = solver.mesh.verts.x.to_numpy()
+    held = np.where(np.linalg.norm(m.verts - np.array([0.0, 1.3, 0.0]), axis=1) < 1e-6)[0]
+    assert np.allclose(x[held], m.verts[held])  # pinned by the hang collider

@@ -387,3 +387,56 @@ def test_stress_run_stays_finite_and_relaxes():
     L = o.log()
     assert np.all(np.isfinite(o.x))
     assert L[-1, 3] < L[0, 3]
+
+
+def _sdf_oracle(sdfs, precision=64):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float32) + 10.0
+    t = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    from oracle import Oracle
+    return Oracle(v, t, sdfs, precision=precision, threads=1)
+
+
+def test_cube_sdf_closed_forms():
+    """CubeSdf (sdf.py:98-124): distance to a box of half size h around c."""
+    o = _sdf_oracle([dict(kind="cube", center=(1.0, 2.0, 3.0), size=(2.0, 4.0, 6.0), fixed=False)])
+    c = np.array([1.0, 2.0, 3.0])
+    d, n, _ = o.sdf_eval(0, c + [1.5, 0, 0])        # 0.5 outside the +x face
+    assert d == pytest.approx(0.5) and np.array_equal(n, [1, 0, 0])
+    d, n, _ = o.sdf_eval(0, c + [0, -2.25, 0])       # 0.25 outside the -y face
+    assert d == pytest.approx(0.25) and np.array_equal(n, [0, -1, 0])
+    d, _, _ = o.sdf_eval(0, c + [2.0, 3.0, 4.0])     # outside a corner: Euclidean
+    assert d == pytest.approx(np.sqrt(1.0 + 1.0 + 1.0))
+    d, n, _ = o.sdf_eval(0, c + [0.2, 0.5, -2.6])    # inside: minus the nearest face distance
+    assert d == pytest.approx(-0.4)
+    assert np.array_equal(n, [0, 0, -1])             # axis of the largest |x - c|, not nearest face
+    d, n, _ = o.sdf_eval(0, c + [0.9, 0.1, 0.1])
+    assert d == pytest.approx(-0.1) and np.array_equal(n, [1, 0, 0])
+
+
+def test_cube_normal_uses_largest_offset_not_nearest_face():
+    """The reference's normal picks the axis of the largest |x - c| (sdf.py:114-123),
+    which for a non-cubic box is not always the nearest face."""
+    o = _sdf_oracle([dict(kind="cube", center=(0.0, 0.0, 0.0), size=(10.0, 1.0, 1.0), fixed=True)])
+    d, n, inside = o.sdf_eval(0, (1.0, 0.45, 0.0))
+    assert d == pytest.approx(-0.05)                  # nearest face is +y
+    assert np.array_equal(n, [1, 0, 0]) and inside    # but the normal is +x; fixed: inside
+
+
+def test_hang_sdf_union_and_sphere0_normal():
+    """HangSdfModel (sdf.py:29-56): min over spheres of sqrt(|q|^2 + eps) - r, and
+    the normal is taken towards sphere 0 whichever sphere is nearest."""
+    pts = [[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]]
+    o = _sdf_oracle([dict(kind="hang", points=pts, radius=0.013)])
+    d, n, inside = o.sdf_eval(0, (1.0, 0.005, 0.0))   # inside sphere 1
+    assert d == pytest.approx(0.005 - 0.013, abs=1e-9) and inside  # always fixed
+    assert n == pytest.approx([1.0, 0.005, 0.0] / np.hypot(1.0, 0.005))  # towards sphere 0
+    d, _, inside = o.sdf_eval(0, (0.5, 0.0, 0.0))
+    assert d == pytest.approx(0.5 - 0.013) and not inside
+
+
+def test_nonfixed_cube_leaves_outward_moving_vertex():
+    o = _sdf_oracle([dict(kind="cube", center=(0.0, 0.0, 0.0), size=(2.0, 2.0, 2.0), fixed=False)])
+    _, n, inside = o.sdf_eval(0, (0.9, 0.0, 0.0), v=(1.0, 0.0, 0.0))   # moving out along +x
+    assert not inside
+    _, n, inside = o.sdf_eval(0, (0.9, 0.0, 0.0), v=(-1.0, 0.0, 0.0))  # moving in
+    assert inside and np.array_equal(n, [1, 0, 0])
