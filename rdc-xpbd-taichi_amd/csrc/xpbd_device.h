@@ -145,7 +145,7 @@ constexpr float kEps = 1e-12f;  // xpbd_gpu/constants.py:3 as an f32 constant
 // consecutive patches of a scene (scene-local patch index / kPartialGroup), so
 // a scene's sums do not depend on how work is split or which scenes share a
 // batch.  The residual kernel's work ranges start at group boundaries.
-constexpr int kPartialGroup = 8;
+constexpr int kPartialGroup = 4;
 
 // Reciprocal and square root on the native gfx950 instructions (v_rcp_f32,
 // v_sqrt_f32: about 1 ulp) instead of the correctly rounded sequences (a
