@@ -740,7 +740,7 @@ int enqueue_iterations(xpbd_solver* s, bool capturing) {
   auto project = [&](int it) -> int {
     if (patch)
       HIP_TRY(xpbd::launch_patch_project(T, s->pitems.p, s->prange_proj.p, s->grid_proj, it,
-                                         s->stream));
+                                         false, s->stream));
     else
       HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
     return XPBD_OK;
@@ -1608,7 +1608,7 @@ int batch_enqueue_substep(xpbd_batch* b) {
   HIP_TRY(xpbd::launch_predict(T, b->vmap.p, b->nvb, bs, 1, 1, P.cheb, b->stream));
   for (int it = 0; it < P.iters; ++it) {
     HIP_TRY(xpbd::launch_patch_project(T, b->pitems.p, b->prange_proj.p, b->grid_proj, it,
-                                       b->stream));
+                                       ns > 1, b->stream));
     HIP_TRY(xpbd::launch_patch_average(T, b->vmap.p, b->nvb, bs, it, b->stream));
     if (P.cheb || P.monitor) {
       HIP_TRY(xpbd::launch_patch_residual(T, b->pitems.p, b->prange_res.p, b->grid_res,

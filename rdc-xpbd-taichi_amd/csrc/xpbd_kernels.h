@@ -17,8 +17,10 @@ hipError_t launch_average(const SceneDev* scenes, const WorkItem* vmap, int nblo
                           hipStream_t st);
 hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nblocks, int bs,
                            hipStream_t st);
+// multi_scene: the patch ranges may cross scenes (batches)
 hipError_t launch_patch_project(const SceneDev* scenes, const PatchItem* items,
-                                const int2* ranges, int nblocks, int it, hipStream_t st);
+                                const int2* ranges, int nblocks, int it, bool multi_scene,
+                                hipStream_t st);
 hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks,
                                 int bs, int it, hipStream_t st);
 hipError_t launch_patch_residual(const SceneDev* scenes, const PatchItem* items,

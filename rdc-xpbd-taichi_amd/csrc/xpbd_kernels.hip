@@ -349,21 +349,37 @@ __device__ __forceinline__ float4 cheb_blend(const SceneDev& S, float4 y4, float
   return make_float4(nw.x, nw.y, nw.z, y4.w);
 }
 
-// Iterate y^it of vertex g as staged by the projection of iteration it.  In
-// the fused sequence y^it is formed here from y^(it-1) (ybuf[(it-1)%3]),
-// x_J^(it-1) (xj), prev = y^(it-2) (or y^0 when it == 1) and omega^(it-1), and
-// the vertex's owner patch stores it in ybuf[it%3]; the other patches holding
-// the vertex compute the same bits.
-__device__ __forceinline__ float4 stage_iterate(const SceneDev& S, int g, int it, bool owner,
-                                                float omega) {
-  if (!S.fused || it == 0) return S.pos[g];
-  const float4 y = S.ybuf[(it - 1) % 3][g];
-  // prev is y^(it-2), or y^(it-1) itself at it == 1; selecting the buffer (not
-  // the value) keeps both loads global
-  const float4 pr = S.ybuf[it == 1 ? 0 : (it - 2) % 3][g];
-  const float4 out = cheb_blend(S, y, S.xj[g], pr, omega);
-  if (owner) S.ybuf[it % 3][g] = out;
-  return out;
+// The inputs of one patch vertex's staged iterate y^it: in the fused sequence
+// y^(it-1) (ybuf[(it-1)%3]), prev = y^(it-2) (or y^0 when it == 1) and
+// x_J^(it-1) (xj), otherwise pos; and the rest position.  They are loaded one
+// patch ahead and blended at staging, so the loads stay in flight while the
+// previous patch is computed.
+struct StageIn {
+  float4 ya, yb, xa, r;
+};
+
+__device__ __forceinline__ void load_stage(const SceneDev& S, int g, int it, bool blend,
+                                           StageIn& in) {
+  if (blend) {
+    in.ya = S.ybuf[(it - 1) % 3][g];
+    // selecting the buffer (not the value) keeps both loads global
+    in.yb = S.ybuf[it == 1 ? 0 : (it - 2) % 3][g];
+    in.xa = S.xj[g];
+  } else {
+    in.ya = S.pos[g];
+  }
+  in.r = S.rest[g];
+}
+
+// y^it of a staged vertex: y^it = blend(y^(it-1), x_J^(it-1), prev, omega^(it-1))
+// when blending, else the loaded position.  The vertex's owner patch stores it
+// in ybuf[it%3]; the other patches holding the vertex compute the same bits.
+__device__ __forceinline__ float4 blend_stage(const SceneDev& S, const StageIn& in, int g, int it,
+                                              bool blend, bool owner, float omega) {
+  if (!blend) return in.ya;
+  const float4 y = cheb_blend(S, in.ya, in.xa, in.yb, omega);
+  if (owner) S.ybuf[it % 3][g] = y;
+  return y;
 }
 
 // ---- per-scene scalar control ----------------------------------------------------
@@ -501,57 +517,47 @@ __device__ __noinline__ float fused_omega(const SceneDev& S, int it, bool writer
 // [ranges[b].x, ranges[b].y): it stages a patch's vertices and rest positions
 // in LDS, projects every tet, accumulates the corrections per patch vertex in
 // LDS, and writes one partial per (patch, vertex) slot.  The loads are
-// software-pipelined: while patch i is computed, the vertex data of patch i+1
-// and the vertex ids of patch i+2 are in flight.  Deterministic mode stores
-// each tet corner at its vertex's rank position in an LDS array sorted by
-// vertex (ranks follow tet order) and sums each vertex's run in that order;
-// atomic mode adds into per-vertex LDS cells.
+// software-pipelined: while patch i is computed, the vertex inputs of patch
+// i+1 and the vertex ids of patch i+2 are in flight.  Deterministic mode
+// stores each tet corner at its vertex's rank position in an LDS array sorted
+// by vertex (ranks follow tet order) and sums each vertex's run in that order;
+// atomic mode adds into per-vertex LDS cells.  kMultiScene: the patch ranges
+// may cross scenes (batches), so omega is re-evaluated on a scene change.
+template <bool kMultiScene>
 __global__ void __launch_bounds__(kPatchThreads)
     k_patch_project(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
                     const int2* __restrict__ ranges, int it) {
   const int2 rg = ranges[blockIdx.x];
   if (rg.x >= rg.y) return;
-  const int first = it == 0;
+  const bool first = it == 0;
   __shared__ float4 s_y[kPatchVerts];
+  // rest positions; w holds the start of the vertex's run in s_sorted
   __shared__ float4 s_rest[kPatchVerts];
-  __shared__ int s_base[kPatchVerts];
   // corner corrections sorted by vertex (deterministic) or per-vertex LDS atomics
   __shared__ float4 s_sorted[4 * kPatchThreads];
   float4* s_acc = s_sorted;
   const int tid = threadIdx.x;
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
 
-  // stage 0 (current): everything in registers
+  // stage 0 (current patch): vertex ids and inputs, tet data
   PatchItem h0 = items[rg.x];
+  const SceneDev& S0 = scenes[h0.scene];
+  bool blend0 = S0.fused && !first;
+  int g0 = 0;
   int2 vs0 = make_int2(0, 0);
-  float4 y0 = make_float4(0.f, 0.f, 0.f, 0.f), r0 = y0;
+  StageIn in0{zero, zero, zero, zero};
   uint2 loc0 = make_uint2(0u, 0u);
   float2 lam0 = make_float2(0.f, 0.f);
-  // omega^(it-1) of the scene whose patches are being staged
-  float omega = 1.f;
-  int omega_scene = -1;
-  // the first patch's loads, the blend inputs included, are issued before the
-  // controller update so that they are in flight while it runs
-  const SceneDev& S0 = scenes[h0.scene];
-  const bool blend0 = S0.fused && !first;
-  int g0 = 0;
-  float4 ya = y0, yb = y0, xa = y0;
   if (tid < h0.n_verts) {
     g0 = S0.p_vert_ids[h0.vert_begin + tid];
     vs0 = S0.p_vert_sum[h0.vert_begin + tid];
-    r0 = S0.rest[g0];
-    if (blend0) {
-      ya = S0.ybuf[(it - 1) % 3][g0];
-      yb = S0.ybuf[it == 1 ? 0 : (it - 2) % 3][g0];
-      xa = S0.xj[g0];
-    } else {
-      ya = S0.pos[g0];
-    }
+    load_stage(S0, g0, it, blend0, in0);
   }
   if (tid < h0.n_tets) {
     loc0 = S0.tet_local[h0.tet_begin + tid];
     if (!first) lam0 = S0.lam[h0.tet_begin + tid];
   }
-  // stage 1 (next): vertex ids only
+  // stage 1 (next patch): vertex ids only
   bool has1 = rg.x + 1 < rg.y;
   PatchItem h1 = h0;
   int g1 = 0;
@@ -564,39 +570,36 @@ __global__ void __launch_bounds__(kPatchThreads)
       vs1 = S.p_vert_sum[h1.vert_begin + tid];
     }
   }
-  y0 = ya;
+  // omega^(it-1) of the scene of the patch staged next, evaluated while the
+  // first patch's loads are in flight
+  float omega = 1.f;
+  int omega_scene = -1;
   if (blend0) {
     omega = scene_omega(S0, it, h0.patch == 0);
     omega_scene = h0.scene;
-    if (tid < h0.n_verts) {  // as stage_iterate
-      y0 = cheb_blend(S0, ya, xa, yb, omega);
-      if (vs0.x < 0) S0.ybuf[it % 3][g0] = y0;
-    }
   }
   for (int i = rg.x; i < rg.y; ++i) {
     const SceneDev& S = scenes[h0.scene];
     const bool csr = S.csr != 0;
     if (tid < h0.n_verts) {
-      s_y[tid] = y0;
-      s_rest[tid] = r0;
-      s_base[tid] = vs0.x & 0xffff;
-      if (!csr) s_acc[tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+      s_y[tid] = blend_stage(S, in0, g0, it, blend0, vs0.x < 0, omega);
+      s_rest[tid] = make_float4(in0.r.x, in0.r.y, in0.r.z, __int_as_float(vs0.x & 0xffff));
+      if (!csr) s_acc[tid] = zero;
     }
     __syncthreads();
-    // in flight during the compute below: data of patch i+1, ids of patch i+2
-    float4 y1 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = y1;
+    // in flight during the compute below: inputs of patch i+1, ids of patch i+2
+    StageIn in1{zero, zero, zero, zero};
+    bool blend1 = false;
     uint2 loc1 = make_uint2(0u, 0u);
     float2 lam1 = make_float2(0.f, 0.f);
     if (has1) {
       const SceneDev& S1 = scenes[h1.scene];
-      if (S1.fused && !first && h1.scene != omega_scene) {  // uniform over the workgroup
+      blend1 = S1.fused && !first;
+      if (kMultiScene && blend1 && h1.scene != omega_scene) {  // uniform over the workgroup
         omega = fused_omega(S1, it, h1.patch == 0);
         omega_scene = h1.scene;
       }
-      if (tid < h1.n_verts) {
-        y1 = stage_iterate(S1, g1, it, vs1.x < 0, omega);
-        r1 = S1.rest[g1];
-      }
+      if (tid < h1.n_verts) load_stage(S1, g1, it, blend1, in1);
       if (tid < h1.n_tets) {
         loc1 = S1.tet_local[h1.tet_begin + tid];
         if (!first) lam1 = S1.lam[h1.tet_begin + tid];
@@ -618,10 +621,10 @@ __global__ void __launch_bounds__(kPatchThreads)
       const int l[4] = {int(loc0.x & 0xffu), int((loc0.x >> 8) & 0xffu),
                         int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
       const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
+      const float4 rs[4] = {s_rest[l[0]], s_rest[l[1]], s_rest[l[2]], s_rest[l[3]]};
       Mat3 idm;
       float inv_det;
-      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm,
-                 inv_det);
+      rest_shape(f3(rs[0]), f3(rs[1]), f3(rs[2]), f3(rs[3]), idm, inv_det);
       float2 lam = lam0;
       TetProjection P;
       project_tet(S, q, idm, inv_det, lam, P);
@@ -630,7 +633,8 @@ __global__ void __launch_bounds__(kPatchThreads)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
           const int rank = int((loc0.y >> (8 * k)) & 0xffu);
-          s_sorted[s_base[l[k]] + rank] = make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
+          s_sorted[__float_as_int(rs[k].w) + rank] =
+              make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
         }
       } else {
 #pragma unroll
@@ -646,7 +650,7 @@ __global__ void __launch_bounds__(kPatchThreads)
     if (tid < h0.n_verts) {
       float4 a;
       if (csr) {
-        a = make_float4(0.f, 0.f, 0.f, 0.f);
+        a = zero;
         const int cb = vs0.x & 0xffff, ce = cb + ((vs0.x >> 16) & 0x7fff);
         int c = cb;
         // four reads in flight, added in the same order as one at a time
@@ -672,9 +676,10 @@ __global__ void __launch_bounds__(kPatchThreads)
     }
     __syncthreads();  // the LDS arrays are refilled by the next patch
     h0 = h1;
+    g0 = g1;
     vs0 = vs1;
-    y0 = y1;
-    r0 = r1;
+    in0 = in1;
+    blend0 = blend1;
     loc0 = loc1;
     lam0 = lam1;
     h1 = h2;
@@ -940,9 +945,13 @@ hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nbl
   return hipGetLastError();
 }
 hipError_t launch_patch_project(const SceneDev* scenes, const PatchItem* items,
-                                const int2* ranges, int nblocks, int it, hipStream_t st) {
+                                const int2* ranges, int nblocks, int it, bool multi_scene,
+                                hipStream_t st) {
   if (nblocks <= 0) return hipSuccess;
-  k_patch_project<<<nblocks, kPatchThreads, 0, st>>>(scenes, items, ranges, it);
+  if (multi_scene)
+    k_patch_project<true><<<nblocks, kPatchThreads, 0, st>>>(scenes, items, ranges, it);
+  else
+    k_patch_project<false><<<nblocks, kPatchThreads, 0, st>>>(scenes, items, ranges, it);
   return hipGetLastError();
 }
 hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks,
@@ -957,8 +966,8 @@ hipError_t launch_patch_residual(const SceneDev* scenes, const PatchItem* items,
   return hipGetLastError();
 }
 hipError_t patch_kernel_occupancy(int* project_blocks_per_cu, int* residual_blocks_per_cu) {
-  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(project_blocks_per_cu,
-                                                              k_patch_project, kPatchThreads, 0);
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      project_blocks_per_cu, k_patch_project<true>, kPatchThreads, 0);
   if (e != hipSuccess) return e;
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(residual_blocks_per_cu, k_patch_residual,
                                                       kPatchThreads, 0);

@@ -111,6 +111,9 @@ class Server:
         self.rebuilds = 0
         self.substeps_run = 0
         self.evictions = 0
+        # host seconds spent building batches and stepping them (launch + wait)
+        self.build_s = 0.0
+        self.step_s = 0.0
 
     @property
     def used_bytes(self) -> int:
@@ -146,24 +149,27 @@ class Server:
 
     def run(self, requests: list, idle_sleep: float = 2e-4) -> list:
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
-        queue = []
+        waiting = {}  # scene -> its arrived requests, oldest first
         active = {}  # scene -> [request, substeps left, start]
         done = []
         batch = None
         key = None
         t0 = time.perf_counter()
-        while pending or queue or active:
+        while pending or waiting or active:
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
-                queue.append(pending.popleft())
-            still = []
-            for req in queue:
-                if req.scene in active or not self.make_resident(req.scene, busy=active):
-                    still.append(req)
+                req = pending.popleft()
+                waiting.setdefault(req.scene, collections.deque()).append(req)
+            # admit the oldest waiting request of every free scene, oldest first
+            for j in sorted((j for j in waiting if j not in active),
+                            key=lambda j: (waiting[j][0].arrival, waiting[j][0].rid)):
+                if not self.make_resident(j, busy=active):
                     continue
-                subs = req.frames * self.specs[req.scene].params["substeps_num"]
-                active[req.scene] = [req, subs, now]
-            queue = still
+                req = waiting[j].popleft()
+                if not waiting[j]:
+                    del waiting[j]
+                subs = req.frames * self.specs[j].params["substeps_num"]
+                active[j] = [req, subs, now]
             if not active:
                 if pending:
                     wait = pending[0].arrival - (time.perf_counter() - t0)
@@ -173,13 +179,17 @@ class Server:
                 raise RuntimeError("queued requests cannot be admitted within the HBM budget")
             k = tuple(sorted(active))
             if k != key:
+                tb = time.perf_counter()
                 if batch is not None:
                     batch.close()
                 batch = SceneBatch([self.resident[j] for j in k], **self.batch_opts)
                 key = k
                 self.rebuilds += 1
+                self.build_s += time.perf_counter() - tb
+            ts = time.perf_counter()
             batch.substep()
             batch.synchronize()
+            self.step_s += time.perf_counter() - ts
             self.substeps_run += 1
             now = time.perf_counter() - t0
             for j in k:
