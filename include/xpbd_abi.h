@@ -44,7 +44,7 @@ extern "C" {
 
 /* layout: how tets are laid out and processed on the device */
 #define XPBD_LAYOUT_FLAT 0  /* one thread per tet, inv_Dm streamed from HBM */
-#define XPBD_LAYOUT_PATCH 1 /* one workgroup per patch of <= 256 tets staged in LDS */
+#define XPBD_LAYOUT_PATCH 1 /* one workgroup per patch of <= 512 tets staged in LDS */
 
 /* kind of an SDF collider (xpbd_gpu/sdf.py) */
 #define XPBD_SDF_PLANE 0  /* PlaneSdf(point, dir, fixed), sdf.py:59-74 */

@@ -79,7 +79,7 @@ struct SceneDev {
   gptr<const float4> dm0;
   gptr<const float4> dm1;
   gptr<const float2> dm2;
-  // patch layout: one workgroup per patch of <= 256 tets / <= 256 vertices
+  // patch layout: one workgroup per patch of <= 512 tets / <= 256 vertices
   gptr<const int> p_tet_begin;   // n_patches + 1
   gptr<const int> p_vert_begin;  // n_patches + 1: range of the patch's vertex records
   gptr<const int> p_vert_ids;    // per patch vertex: global vertex id
@@ -141,11 +141,20 @@ struct __align__(32) PatchItem {
 
 constexpr float kEps = 1e-12f;  // xpbd_gpu/constants.py:3 as an f32 constant
 
+// Patch shape, shared by the host's patch builder and the patch kernels: a
+// patch holds at most kPatchTets consecutive tets (kPatchTetsPerThread per
+// thread of a kPatchThreads workgroup) touching at most kPatchVerts vertices
+// (one per thread; local ids and corner ranks fit in a byte).
+constexpr int kPatchThreads = 256;
+constexpr int kPatchTetsPerThread = 2;
+constexpr int kPatchTets = kPatchThreads * kPatchTetsPerThread;
+constexpr int kPatchVerts = 256;
+
 // Residual partial sums of the patch layout: one per group of kPartialGroup
 // consecutive patches of a scene (scene-local patch index / kPartialGroup), so
 // a scene's sums do not depend on how work is split or which scenes share a
 // batch.  The residual kernel's work ranges start at group boundaries.
-constexpr int kPartialGroup = 4;
+constexpr int kPartialGroup = 2;
 
 // Reciprocal and square root on the native gfx950 instructions (v_rcp_f32,
 // v_sqrt_f32: about 1 ulp) instead of the correctly rounded sequences (a

@@ -31,6 +31,8 @@ using xpbd::SceneDev;
 using xpbd::SdfDev;
 using xpbd::WorkItem;
 using xpbd::PatchItem;
+using xpbd::kPatchTets;
+using xpbd::kPatchVerts;
 
 namespace {
 
@@ -168,8 +170,6 @@ int validate_sdfs(const xpbd_sdf* sdfs, int n) {
 // kPatchTets tets touching at most kPatchVerts vertices (local ids and corner
 // ranks fit in a byte).  Each (patch, vertex) pair is a slot;
 // a vertex's slots are listed in ascending patch order.
-constexpr int kPatchTets = 256;   // one tet per thread of a 256-thread workgroup
-constexpr int kPatchVerts = 256;  // one vertex per thread; u8 local ids
 
 struct Patches {
   std::vector<int> tet_begin, vert_begin, vert_ids, vslot_off;

@@ -9,20 +9,17 @@
 // Two tet layouts share the per-tet math:
 //  flat   one thread per tet; inv_Dm/V0 streamed from HBM; corrections go to
 //         per-vertex float atomics or to per-corner slots gathered per vertex.
-//  patch  one workgroup per patch (<= 256 Morton-contiguous tets, <= 256
-//         vertices); the patch's vertices are staged in LDS, inv_Dm/V0 are
-//         recomputed from the staged rest positions, corrections accumulate in
-//         LDS and leave the workgroup as one partial per (patch, vertex) slot,
-//         so no global atomics are issued.
+//  patch  one workgroup per patch (<= 512 bisection-ordered tets, two per
+//         thread, <= 256 vertices); the patch's vertices are staged in LDS,
+//         inv_Dm/V0 are recomputed from the staged rest positions, corrections
+//         accumulate in LDS and leave the workgroup as one partial per
+//         (patch, vertex) slot, so no global atomics are issued.
 #include "xpbd_device.h"
 #include "xpbd_kernels.h"
 
 namespace xpbd {
 
 namespace {
-
-constexpr int kPatchThreads = 256;  // threads per patch workgroup = max tets per patch
-constexpr int kPatchVerts = 256;    // max vertices per patch (u8 local ids)
 
 __device__ __forceinline__ void atomic_add3(float4* dst, float3 v) {
   unsafeAtomicAdd(&dst->x, v.x);
@@ -388,13 +385,18 @@ __device__ __forceinline__ float4 blend_stage(const SceneDev& S, const StageIn& 
 // the one folded into the projection sum in the same order.
 constexpr int kReduceThreads = kPatchThreads;
 
+// LDS doubles a scene reduction needs as scratch.
+constexpr int kReduceScratch = 3 * (kReduceThreads / 64);
+
 // Fixed-order sums of a scene's partials (component 0, and 1-2 when `all3`)
 // by one workgroup of kReduceThreads threads: each thread adds its strided
 // entries in ascending order, then waves and workgroup combine in a fixed
 // tree.  Every workgroup that calls it gets the same bits; thread 0 gets the
-// totals.  Ends with a workgroup barrier.
-__device__ __forceinline__ void scene_sums(const SceneDev& S, bool all3, double tot[3]) {
-  __shared__ double red[3][kReduceThreads / 64];
+// totals.  `red` is LDS scratch of kReduceScratch doubles.  Ends with a
+// workgroup barrier.
+__device__ __forceinline__ void scene_sums(const SceneDev& S, bool all3, double tot[3],
+                                           double* red) {
+  constexpr int kW = kReduceThreads / 64;
   const int n = S.n_partials;
   const int tid = threadIdx.x;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -418,17 +420,17 @@ __device__ __forceinline__ void scene_sums(const SceneDev& S, bool all3, double 
   }
   const int lane = tid & 63, wave = tid >> 6;
   if (lane == 0) {
-    red[0][wave] = acc[0];
-    red[1][wave] = acc[1];
-    red[2][wave] = acc[2];
+    red[wave] = acc[0];
+    red[kW + wave] = acc[1];
+    red[2 * kW + wave] = acc[2];
   }
   __syncthreads();
   if (tid == 0) {
     tot[0] = tot[1] = tot[2] = 0.0;
-    for (int w = 0; w < kReduceThreads / 64; ++w) {
-      tot[0] += red[0][w];
-      tot[1] += red[1][w];
-      tot[2] += red[2][w];
+    for (int w = 0; w < kW; ++w) {
+      tot[0] += red[w];
+      tot[1] += red[kW + w];
+      tot[2] += red[2 * kW + w];
     }
   }
   __syncthreads();
@@ -490,39 +492,64 @@ __device__ __forceinline__ void store_log_row(const SceneDev& S, int it, const d
 // launch is needed only after the last iteration; the workgroup holding the
 // scene's patch 0 stores the new state and the log row, into the slot no
 // workgroup of this launch reads.  Returns omega^(it-1) to every thread.
-__device__ __forceinline__ float scene_omega(const SceneDev& S, int it, bool writer) {
-  __shared__ float s_omega;
+// `red` is LDS scratch of kReduceScratch + 1 doubles; it is read after the
+// last barrier, so the caller separates its next writes there by a barrier.
+__device__ __forceinline__ float scene_omega(const SceneDev& S, int it, bool writer,
+                                             double* red) {
   double tot[3];
-  scene_sums(S, S.monitor != 0, tot);
+  scene_sums(S, S.monitor != 0, tot, red);
+  float* s_omega = reinterpret_cast<float*>(red + kReduceScratch);
   if (threadIdx.x == 0) {
     double L[6];
     const SceneCtrl c = ctrl_update(S, S.ctrl[it & 1], it - 1, tot, L);
-    s_omega = float(c.omega);
+    *s_omega = float(c.omega);
     if (writer) {
       S.ctrl[(it - 1) & 1] = c;
       store_log_row(S, it - 1, L);
     }
   }
   __syncthreads();
-  return s_omega;
+  return *s_omega;
 }
 
 // Out-of-line form for the projection's scene changes inside its loop (batches
 // only), which keeps the loop's register use down.
-__device__ __noinline__ float fused_omega(const SceneDev& S, int it, bool writer) {
-  return scene_omega(S, it, writer);
+__device__ __noinline__ float fused_omega(const SceneDev& S, int it, bool writer, double* red) {
+  return scene_omega(S, it, writer, red);
+}
+
+// Tet data of one patch held by a thread: tet tid + j * kPatchThreads for j
+// in [0, kPatchTetsPerThread) (local ids and corner ranks, multipliers).
+struct PatchTets {
+  uint2 loc[kPatchTetsPerThread];
+  float2 lam[kPatchTetsPerThread];
+};
+
+__device__ __forceinline__ void load_patch_tets(const SceneDev& S, const PatchItem& h, bool lam,
+                                                PatchTets& d) {
+#pragma unroll
+  for (int j = 0; j < kPatchTetsPerThread; ++j) {
+    const int t = threadIdx.x + j * kPatchThreads;
+    d.loc[j] = make_uint2(0u, 0u);
+    d.lam[j] = make_float2(0.f, 0.f);
+    if (t < h.n_tets) {
+      d.loc[j] = S.tet_local[h.tet_begin + t];
+      if (lam) d.lam[j] = S.lam[h.tet_begin + t];
+    }
+  }
 }
 
 // Persistent patch projection.  Workgroup b walks the patch items
 // [ranges[b].x, ranges[b].y): it stages a patch's vertices and rest positions
 // in LDS, projects every tet, accumulates the corrections per patch vertex in
 // LDS, and writes one partial per (patch, vertex) slot.  The loads are
-// software-pipelined: while patch i is computed, the vertex inputs of patch
-// i+1 and the vertex ids of patch i+2 are in flight.  Deterministic mode
-// stores each tet corner at its vertex's rank position in an LDS array sorted
-// by vertex (ranks follow tet order) and sums each vertex's run in that order;
-// atomic mode adds into per-vertex LDS cells.  kMultiScene: the patch ranges
-// may cross scenes (batches), so omega is re-evaluated on a scene change.
+// software-pipelined: while patch i is computed, the vertex inputs and tet
+// data of patch i+1 and the vertex ids of patch i+2 are in flight.
+// Deterministic mode stores each tet corner at its vertex's rank position in
+// an LDS array sorted by vertex (ranks follow tet order) and sums each
+// vertex's run in that order; atomic mode adds into per-vertex LDS cells.
+// kMultiScene: the patch ranges may cross scenes (batches), so omega is
+// re-evaluated on a scene change.
 template <bool kMultiScene>
 __global__ void __launch_bounds__(kPatchThreads)
     k_patch_project(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
@@ -533,9 +560,11 @@ __global__ void __launch_bounds__(kPatchThreads)
   __shared__ float4 s_y[kPatchVerts];
   // rest positions; w holds the start of the vertex's run in s_sorted
   __shared__ float4 s_rest[kPatchVerts];
-  // corner corrections sorted by vertex (deterministic) or per-vertex LDS atomics
-  __shared__ float4 s_sorted[4 * kPatchThreads];
+  // corner corrections sorted by vertex (deterministic) or per-vertex LDS
+  // atomics; between patches also the scratch of the omega evaluation
+  __shared__ float4 s_sorted[4 * kPatchTets];
   float4* s_acc = s_sorted;
+  double* red = reinterpret_cast<double*>(s_sorted);
   const int tid = threadIdx.x;
   const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
 
@@ -546,17 +575,13 @@ __global__ void __launch_bounds__(kPatchThreads)
   int g0 = 0;
   int2 vs0 = make_int2(0, 0);
   StageIn in0{zero, zero, zero, zero};
-  uint2 loc0 = make_uint2(0u, 0u);
-  float2 lam0 = make_float2(0.f, 0.f);
+  PatchTets t0;
   if (tid < h0.n_verts) {
     g0 = S0.p_vert_ids[h0.vert_begin + tid];
     vs0 = S0.p_vert_sum[h0.vert_begin + tid];
     load_stage(S0, g0, it, blend0, in0);
   }
-  if (tid < h0.n_tets) {
-    loc0 = S0.tet_local[h0.tet_begin + tid];
-    if (!first) lam0 = S0.lam[h0.tet_begin + tid];
-  }
+  load_patch_tets(S0, h0, !first, t0);
   // stage 1 (next patch): vertex ids only
   bool has1 = rg.x + 1 < rg.y;
   PatchItem h1 = h0;
@@ -571,11 +596,12 @@ __global__ void __launch_bounds__(kPatchThreads)
     }
   }
   // omega^(it-1) of the scene of the patch staged next, evaluated while the
-  // first patch's loads are in flight
+  // first patch's loads are in flight (the staging below writes no LDS the
+  // evaluation reads, and a barrier follows it)
   float omega = 1.f;
   int omega_scene = -1;
   if (blend0) {
-    omega = scene_omega(S0, it, h0.patch == 0);
+    omega = scene_omega(S0, it, h0.patch == 0, red);
     omega_scene = h0.scene;
   }
   for (int i = rg.x; i < rg.y; ++i) {
@@ -590,20 +616,17 @@ __global__ void __launch_bounds__(kPatchThreads)
     // in flight during the compute below: inputs of patch i+1, ids of patch i+2
     StageIn in1{zero, zero, zero, zero};
     bool blend1 = false;
-    uint2 loc1 = make_uint2(0u, 0u);
-    float2 lam1 = make_float2(0.f, 0.f);
+    PatchTets t1;
     if (has1) {
       const SceneDev& S1 = scenes[h1.scene];
       blend1 = S1.fused && !first;
       if (kMultiScene && blend1 && h1.scene != omega_scene) {  // uniform over the workgroup
-        omega = fused_omega(S1, it, h1.patch == 0);
+        omega = fused_omega(S1, it, h1.patch == 0, red);
         omega_scene = h1.scene;
+        __syncthreads();  // omega is read before s_sorted is written again
       }
       if (tid < h1.n_verts) load_stage(S1, g1, it, blend1, in1);
-      if (tid < h1.n_tets) {
-        loc1 = S1.tet_local[h1.tet_begin + tid];
-        if (!first) lam1 = S1.lam[h1.tet_begin + tid];
-      }
+      load_patch_tets(S1, h1, !first, t1);
     }
     const bool has2 = i + 2 < rg.y;
     PatchItem h2 = h1;
@@ -617,32 +640,37 @@ __global__ void __launch_bounds__(kPatchThreads)
         vs2 = S2.p_vert_sum[h2.vert_begin + tid];
       }
     }
-    if (tid < h0.n_tets) {
-      const int l[4] = {int(loc0.x & 0xffu), int((loc0.x >> 8) & 0xffu),
-                        int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
-      const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
-      const float4 rs[4] = {s_rest[l[0]], s_rest[l[1]], s_rest[l[2]], s_rest[l[3]]};
-      Mat3 idm;
-      float inv_det;
-      rest_shape(f3(rs[0]), f3(rs[1]), f3(rs[2]), f3(rs[3]), idm, inv_det);
-      float2 lam = lam0;
-      TetProjection P;
-      project_tet(S, q, idm, inv_det, lam, P);
-      S.lam[h0.tet_begin + tid] = lam;
-      if (csr) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          const int rank = int((loc0.y >> (8 * k)) & 0xffu);
-          s_sorted[__float_as_int(rs[k].w) + rank] =
-              make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
-        }
-      } else {
+    for (int j = 0; j < kPatchTetsPerThread; ++j) {
+      const int t = tid + j * kPatchThreads;
+      if (t < h0.n_tets) {
+        const uint2 loc = t0.loc[j];
+        const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu),
+                          int((loc.x >> 16) & 0xffu), int(loc.x >> 24)};
+        const float4 q[4] = {s_y[l[0]], s_y[l[1]], s_y[l[2]], s_y[l[3]]};
+        const float4 rs[4] = {s_rest[l[0]], s_rest[l[1]], s_rest[l[2]], s_rest[l[3]]};
+        Mat3 idm;
+        float inv_det;
+        rest_shape(f3(rs[0]), f3(rs[1]), f3(rs[2]), f3(rs[3]), idm, inv_det);
+        float2 lam = t0.lam[j];
+        TetProjection P;
+        project_tet(S, q, idm, inv_det, lam, P);
+        S.lam[h0.tet_begin + t] = lam;
+        if (csr) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-          atomicAdd(&s_acc[l[k]].x, P.dp[k].x);
-          atomicAdd(&s_acc[l[k]].y, P.dp[k].y);
-          atomicAdd(&s_acc[l[k]].z, P.dp[k].z);
-          if (!S.block_mode) atomicAdd(&s_acc[l[k]].w, P.cnt[k]);
+          for (int k = 0; k < 4; ++k) {
+            const int rank = int((loc.y >> (8 * k)) & 0xffu);
+            s_sorted[__float_as_int(rs[k].w) + rank] =
+                make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            atomicAdd(&s_acc[l[k]].x, P.dp[k].x);
+            atomicAdd(&s_acc[l[k]].y, P.dp[k].y);
+            atomicAdd(&s_acc[l[k]].z, P.dp[k].z);
+            if (!S.block_mode) atomicAdd(&s_acc[l[k]].w, P.cnt[k]);
+          }
         }
       }
     }
@@ -680,8 +708,7 @@ __global__ void __launch_bounds__(kPatchThreads)
     vs0 = vs1;
     in0 = in1;
     blend0 = blend1;
-    loc0 = loc1;
-    lam0 = lam1;
+    t0 = t1;
     h1 = h2;
     g1 = g2;
     vs1 = vs2;
@@ -725,8 +752,7 @@ __global__ void __launch_bounds__(kPatchThreads)
   const int tid = threadIdx.x;
   PatchItem h0 = items[rg.x];
   float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), r0 = x0;
-  uint2 loc0 = make_uint2(0u, 0u);
-  float2 lam0 = make_float2(0.f, 0.f);
+  PatchTets t0;
   {
     const SceneDev& S = scenes[h0.scene];
     if (tid < h0.n_verts) {
@@ -734,10 +760,7 @@ __global__ void __launch_bounds__(kPatchThreads)
       x0 = stage_jacobi(S, g);
       r0 = S.rest[g];
     }
-    if (tid < h0.n_tets) {
-      loc0 = S.tet_local[h0.tet_begin + tid];
-      lam0 = S.lam[h0.tet_begin + tid];
-    }
+    load_patch_tets(S, h0, true, t0);
   }
   bool has1 = rg.x + 1 < rg.y;
   PatchItem h1 = h0;
@@ -757,18 +780,14 @@ __global__ void __launch_bounds__(kPatchThreads)
     }
     __syncthreads();
     float4 x1 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = x1;
-    uint2 loc1 = make_uint2(0u, 0u);
-    float2 lam1 = make_float2(0.f, 0.f);
+    PatchTets t1;
     if (has1) {
       const SceneDev& S1 = scenes[h1.scene];
       if (tid < h1.n_verts) {
         x1 = stage_jacobi(S1, g1);
         r1 = S1.rest[g1];
       }
-      if (tid < h1.n_tets) {
-        loc1 = S1.tet_local[h1.tet_begin + tid];
-        lam1 = S1.lam[h1.tet_begin + tid];
-      }
+      load_patch_tets(S1, h1, true, t1);
     }
     const bool has2 = i + 2 < rg.y;
     PatchItem h2 = h1;
@@ -777,19 +796,23 @@ __global__ void __launch_bounds__(kPatchThreads)
       h2 = items[i + 2];
       if (tid < h2.n_verts) g2 = scenes[h2.scene].p_vert_ids[h2.vert_begin + tid];
     }
-    if (tid < h0.n_tets) {
-      const int l[4] = {int(loc0.x & 0xffu), int((loc0.x >> 8) & 0xffu),
-                        int((loc0.x >> 16) & 0xffu), int(loc0.x >> 24)};
-      Mat3 idm;
-      float inv_det;
-      rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm,
-                 inv_det);
-      float r, ah, ad;
-      residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, inv_det, lam0,
-                   r, ah, ad);
-      acc_r += r;
-      acc_h += ah;
-      acc_d += ad;
+#pragma unroll
+    for (int j = 0; j < kPatchTetsPerThread; ++j) {
+      if (tid + j * kPatchThreads < h0.n_tets) {
+        const uint2 loc = t0.loc[j];
+        const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu),
+                          int((loc.x >> 16) & 0xffu), int(loc.x >> 24)};
+        Mat3 idm;
+        float inv_det;
+        rest_shape(f3(s_rest[l[0]]), f3(s_rest[l[1]]), f3(s_rest[l[2]]), f3(s_rest[l[3]]), idm,
+                   inv_det);
+        float r, ah, ad;
+        residual_tet(S, f3(s_x[l[0]]), f3(s_x[l[1]]), f3(s_x[l[2]]), f3(s_x[l[3]]), idm, inv_det,
+                     t0.lam[j], r, ah, ad);
+        acc_r += r;
+        acc_h += ah;
+        acc_d += ad;
+      }
     }
     if ((h0.patch + 1) % kPartialGroup == 0 || h0.patch + 1 == S.n_patches) {  // uniform
       double tot[3];
@@ -801,8 +824,7 @@ __global__ void __launch_bounds__(kPatchThreads)
     h0 = h1;
     x0 = x1;
     r0 = r1;
-    loc0 = loc1;
-    lam0 = lam1;
+    t0 = t1;
     h1 = h2;
     g1 = g2;
     has1 = has2;
@@ -814,10 +836,11 @@ __global__ void __launch_bounds__(kPatchThreads)
 // the iteration's log row.
 __global__ void __launch_bounds__(kReduceThreads) k_reduce(const SceneDev* __restrict__ scenes,
                                                            int it) {
+  __shared__ double red[kReduceScratch];
   const SceneDev& S = scenes[blockIdx.x];
   if (it >= S.iters) return;
   double tot[3];
-  scene_sums(S, true, tot);
+  scene_sums(S, true, tot, red);
   if (threadIdx.x != 0) return;
   double L[6];
   const SceneCtrl c = ctrl_update(S, S.ctrl[(it + 1) & 1], it, tot, L);

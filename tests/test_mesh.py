@@ -70,7 +70,7 @@ def test_reads_reference_assets():
 
 
 def test_patch_builder_invariants():
-    """Patches hold <= 256 tets and <= 256 vertices, every tet's local ids map
+    """Patches hold <= 512 tets and <= 256 vertices, every tet's local ids map
     back to its vertices (checked inside the library), and slots cover each
     vertex once per patch that touches it."""
     from xpbd_gpu import _abi
@@ -79,8 +79,8 @@ def test_patch_builder_invariants():
     for m in (box_grid(1), box_grid(5, 3, 7), box_grid(16, jitter=0.2, seed=2)):
         for pos in (None, m.verts):
             st = _abi.patch_stats(m.tets, m.n_verts, pos)
-            assert st["max_patch_tets"] <= 256 and st["max_patch_verts"] <= 256
-            assert st["n_patches"] >= (m.n_tets + 255) // 256
+            assert st["max_patch_tets"] <= 512 and st["max_patch_verts"] <= 256
+            assert st["n_patches"] >= (m.n_tets + 511) // 512
             assert st["n_slots"] >= m.n_verts
     with pytest.raises(ValueError):
         _abi.patch_stats(np.array([[0, 1, 2, 9]]), 4)
