@@ -405,17 +405,16 @@ __device__ __forceinline__ void scene_sums(const SceneDev& S, bool all3, double 
     if (c > 0 && !all3) break;
     gptr<const double> p = S.partial + size_t(c) * size_t(n);
     double sum = 0.0;
-    int b = tid;
-    // four independent loads in flight per step
-    for (; b + 3 * kReduceThreads < n; b += 4 * kReduceThreads) {
-      const double a0 = p[b], a1 = p[b + kReduceThreads], a2 = p[b + 2 * kReduceThreads],
-                   a3 = p[b + 3 * kReduceThreads];
-      sum += a0;
-      sum += a1;
-      sum += a2;
-      sum += a3;
+    // eight loads in flight per step, all issued before the first add (the
+    // indices are clamped, not branched on), then added in ascending order
+    for (int b = tid; b < n; b += 8 * kReduceThreads) {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = p[min(b + k * kReduceThreads, n - 1)];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (b + k * kReduceThreads < n) sum += v[k];
     }
-    for (; b < n; b += kReduceThreads) sum += p[b];
     acc[c] = wave_sum_d(sum);
   }
   const int lane = tid & 63, wave = tid >> 6;
