@@ -95,8 +95,8 @@ def summarize_pmc(folder: str, copy_mib: int = 512) -> str:
         rows.append((b * len(f.get(k, [])), k, len(f.get(k, [])), fv, wv, b))
     for _, k, n, fv, wv, b in sorted(rows, reverse=True):
         lines.append(f"{k:28s} {n:10d} {fv:12.1f} {wv:12.1f} {b:24.0f}")
-    proj = next(r for r in rows if r[1] == "xpbd::k_patch_project")
-    lines += ["", f"traffic_bytes_per_launch xpbd::k_patch_project: {proj[5]:.0f}"]
+    proj = next(r for r in rows if r[1].startswith("xpbd::k_patch_project"))
+    lines += ["", f"traffic_bytes_per_launch {proj[1]}: {proj[5]:.0f}"]
     return "\n".join(lines) + "\n"
 
 
@@ -129,8 +129,8 @@ def summarize_stats(folder: str) -> str:
     graph = proj[:4000]
     if graph:
         avg = sum(e - b for b, e in graph) / len(graph) / 1e3
-        lines += ["", f"xpbd::k_patch_project over the {len(graph)} graph-replayed launches: "
-                      f"avg {avg:.2f} us"]
+        lines += ["", f"xpbd::k_patch_project<false> over the {len(graph)} graph-replayed "
+                      f"launches: avg {avg:.2f} us"]
     return "\n".join(lines) + "\n"
 
 
@@ -140,7 +140,7 @@ def committed_traffic(path: str = PMC_SUMMARY):
     try:
         with open(path) as fh:
             for line in fh:
-                if line.startswith("traffic_bytes_per_launch xpbd::k_patch_project:"):
+                if line.startswith("traffic_bytes_per_launch xpbd::k_patch_project"):
                     return int(float(line.rsplit(":", 1)[1]))
     except OSError:
         pass
@@ -385,7 +385,8 @@ def main():
     traffic = committed_traffic() if (args.n == 64 and args.layout == "patch") else None
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                "kernel": "xpbd::k_patch_project" if args.layout == "patch" else "xpbd::k_project",
+                "kernel": "xpbd::k_patch_project<false>" if args.layout == "patch"
+                          else "xpbd::k_project",
                 "avg_us": round(avg_s * 1e6, 2), "launches": launches,
                 "avg_us_one_launch_per_event_pair": round(avg_single_s * 1e6, 2),
                 "bytes_per_launch": b_alg_project(T, V)}
