@@ -59,12 +59,15 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
   const float aH = S.alpha_kh * inv_det;
   if (S.block_mode) {
     const float C_D = norm_sqr(F) - 3.f;
-    const float C_H = det3(F) - 1.f;
+    // det F by expansion along the first column, from the cofactor matrix
+    // the gradient needs anyway (the same products as det3)
+    const Mat3 H = ddet_dF(F);
+    const float C_H = F.m[0][0] * H.m[0][0] + F.m[1][0] * H.m[1][0] + F.m[2][0] * H.m[2][0] - 1.f;
     // grad C_D = 2 F inv_Dm^T: the gradients gF of F inv_Dm^T are formed and
     // the factor 2 is applied to the scalars below (exact in floating point)
     float3 gF[4], gH[4];
     vertex_grads(F, idm, gF);
-    vertex_grads(ddet_dF(F), idm, gH);
+    vertex_grads(H, idm, gH);
     const float sFF = w[0] * dot(gF[0], gF[0]) + w[1] * dot(gF[1], gF[1]) +
                       w[2] * dot(gF[2], gF[2]) + w[3] * dot(gF[3], gF[3]);
     const float sHH = w[0] * dot(gH[0], gH[0]) + w[1] * dot(gH[1], gH[1]) +
@@ -83,7 +86,7 @@ __device__ __forceinline__ void project_tet(const SceneDev& S, const float4 q[4]
     const float dlD2 = 2.f * dlD;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      out.dp[k] = w[k] * (gF[k] * dlD2 + gH[k] * dlH);
+      out.dp[k] = gF[k] * (w[k] * dlD2) + gH[k] * (w[k] * dlH);
       out.cnt[k] = 1.f;
     }
   } else {
