@@ -169,12 +169,19 @@ def cpu_baseline(scene, substeps: int, threads: int) -> dict:
 def batched_measurement(n_scenes: int, frames: int, tune: bool) -> dict:
     """256 ragged scenes in one batched launch sequence vs the same scenes
     solved one after another with plain per-scene solves."""
-    from xpbd_gpu.batch import SceneBatch, autotune, default_candidates
+    from xpbd_gpu.batch import SceneBatch, autotune, autotune_ordering, default_candidates
     from xpbd_gpu.scenes import batched_scenes, make_solver
 
     scenes = batched_scenes(n_scenes=n_scenes)
+    order, order_trials = "bisect", []
+    if tune:  # the tet ordering is fixed at creation: pick it first
+        order, order_trials = autotune_ordering(
+            lambda o: [make_solver(s, reorder=o) for s in scenes])
+        print(f"[bench] batched: ordering trials "
+              f"{[(o, round(t * 1e3, 2)) for o, t in order_trials]} ms/substep",
+              file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    solvers = [make_solver(s) for s in scenes]
+    solvers = [make_solver(s, reorder=order) for s in scenes]
     setup = time.perf_counter() - t0
     T = sum(s.n_tets for s in solvers)
     iters = scenes[0].params["constraints_iter"]
@@ -219,8 +226,9 @@ def batched_measurement(n_scenes: int, frames: int, tune: bool) -> dict:
         "sequential_frame_ms_per_scene": {"p50": float(np.percentile(seq_lat, 50) * 1e3),
                                           "p99": float(np.percentile(seq_lat, 99) * 1e3)},
         "sequential_all_scenes_ms": seq_el * 1e3,
-        "tuned_opts": opts,
-        "tuning_trials": len(trials),
+        "tuned_opts": dict(opts, tet_order=order),
+        "tuning_trials": len(trials) + len(order_trials),
+        "order_trials_ms_per_substep": {o: round(t * 1e3, 3) for o, t in order_trials},
     }
 
 

@@ -46,6 +46,18 @@ extern "C" {
 #define XPBD_LAYOUT_FLAT 0  /* one thread per tet, inv_Dm streamed from HBM */
 #define XPBD_LAYOUT_PATCH 1 /* one workgroup per patch of <= 512 tets staged in LDS */
 
+/* Internal tet and vertex order (xpbd_desc.reorder; results are returned in
+ * the caller's order whatever is chosen):
+ *  NONE    the caller's order; patches are cut greedily along it.
+ *  BISECT  patch layout: coordinate bisection of the tet centroids into
+ *          compact patches, vertices numbered by first appearance; flat
+ *          layout: Morton order of tets and vertices.
+ *  MORTON  Morton order of tet centroids and vertices; patches are cut
+ *          greedily along it. */
+#define XPBD_ORDER_NONE 0
+#define XPBD_ORDER_BISECT 1
+#define XPBD_ORDER_MORTON 2
+
 /* kind of an SDF collider (xpbd_gpu/sdf.py) */
 #define XPBD_SDF_PLANE 0  /* PlaneSdf(point, dir, fixed), sdf.py:59-74 */
 #define XPBD_SDF_SPHERE 1 /* SphereSdf(center, radius, fixed), sdf.py:77-95 */
@@ -94,7 +106,7 @@ typedef struct xpbd_desc {
   const xpbd_sdf* sdfs;
   /* execution options (no reference counterpart) */
   int32_t accumulate_mode;     /* XPBD_ACCUM_CSR (default) or XPBD_ACCUM_ATOMIC */
-  int32_t reorder;             /* 1: Morton-reorder tets and vertices internally */
+  int32_t reorder;             /* XPBD_ORDER_* internal tet/vertex order (default BISECT) */
   int32_t block_size;          /* threads per workgroup for flat tet kernels (64..1024) */
   int32_t device;              /* HIP device ordinal */
   int32_t layout;              /* XPBD_LAYOUT_PATCH (default) or XPBD_LAYOUT_FLAT */

@@ -492,6 +492,9 @@ int validate_params(const xpbd_desc* d) {
   if (!(d->frame_dt > 0) || !(d->dt > 0)) return set_error(XPBD_EINVAL, "frame_dt and dt must be > 0");
   if (d->layout != XPBD_LAYOUT_FLAT && d->layout != XPBD_LAYOUT_PATCH)
     return set_error(XPBD_EINVAL, "unknown layout %d", d->layout);
+  if (d->reorder != XPBD_ORDER_NONE && d->reorder != XPBD_ORDER_BISECT &&
+      d->reorder != XPBD_ORDER_MORTON)
+    return set_error(XPBD_EINVAL, "unknown order %d", d->reorder);
   return XPBD_OK;
 }
 
@@ -952,7 +955,7 @@ int xpbd_desc_init(xpbd_desc* d) {
   d->gravity[1] = -9.8;
   d->total_mass = 1.0;
   d->accumulate_mode = XPBD_ACCUM_CSR;
-  d->reorder = 1;
+  d->reorder = XPBD_ORDER_BISECT;
   d->block_size = 256;
   d->device = 0;
   d->layout = XPBD_LAYOUT_PATCH;
@@ -1024,7 +1027,7 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
                      [&](int a, int b) { return tc[a] < tc[b]; });
   }
   std::vector<int> patch_cuts;
-  if (d->reorder && d->layout == XPBD_LAYOUT_PATCH) {
+  if (d->reorder == XPBD_ORDER_BISECT && d->layout == XPBD_LAYOUT_PATCH) {
     // compact patches by coordinate bisection; vertices numbered by their first
     // appearance in patch order, so a patch's vertices are nearly contiguous
     std::vector<float> cent(3 * size_t(nt));

@@ -35,6 +35,9 @@ PATH_KERNELS = 0
 PATH_GRAPH = 1
 LAYOUT_FLAT = 0
 LAYOUT_PATCH = 1
+ORDER_NONE = 0
+ORDER_BISECT = 1
+ORDER_MORTON = 2
 
 # Every symbol the header declares; tests check that the library exports them.
 EXPORTED = (

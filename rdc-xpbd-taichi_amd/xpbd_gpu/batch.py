@@ -154,6 +154,34 @@ def autotune(solvers, candidates=None, substeps: int | None = None, warmup: int 
     return best, results
 
 
+def autotune_ordering(build, orders=("bisect", "morton", "none"), opts=None,
+                      substeps: int = 2, warmup: int = 1):
+    """Picks the internal tet/vertex ordering of the scenes: for each order,
+    build(order) creates the scenes (a list of solvers, with reorder=order),
+    a batch with launch options `opts` (default DEFAULT_OPTS) runs `warmup`
+    then `substeps` timed substeps, and the scenes are freed.  Returns
+    (best_order, [(order, seconds_per_substep), ...])."""
+    opts = dict(DEFAULT_OPTS if opts is None else opts)
+    opts["graph_substeps"] = 1
+    results = []
+    for order in orders:
+        solvers = build(order)
+        try:
+            with SceneBatch(solvers, **opts) as b:
+                b.substep(max(warmup, 0))
+                b.synchronize()
+                t0 = time.perf_counter()
+                b.substep(substeps)
+                b.synchronize()
+                results.append((order, (time.perf_counter() - t0) / max(1, substeps)))
+        finally:
+            for s in solvers:
+                s.close()
+            del solvers
+    best = min(results, key=lambda r: r[1])[0]
+    return best, results
+
+
 def run_sequential(solvers, frames: int = 1):
     """The same scenes solved one after another with plain per-scene solves."""
     for s in solvers:
@@ -167,5 +195,5 @@ def scene_bytes(solver) -> int:
     return int(b.value)
 
 
-__all__ = ["SceneBatch", "autotune", "default_candidates", "run_sequential", "scene_bytes",
-           "DEFAULT_OPTS"]
+__all__ = ["SceneBatch", "autotune", "autotune_ordering", "default_candidates",
+           "run_sequential", "scene_bytes", "DEFAULT_OPTS"]

@@ -12,7 +12,9 @@ needs no host round trip.
 Execution options with no reference counterpart are keyword-only:
   accumulate_mode  "csr" (deterministic fixed-order sums, default) or "atomic"
                    (float atomics, the reference's unordered accumulation)
-  reorder          Morton-reorder tets and vertices internally (default True)
+  reorder          internal tet/vertex order: "bisect" (compact patches by
+                   coordinate bisection; True, the default), "morton" or
+                   "none" (the caller's order; False)
   device           HIP device ordinal
   exec_path        "graph" (hipGraph per substep) or "kernels"
   layout           "patch" (workgroup per LDS-staged patch, default) or "flat"
@@ -42,6 +44,8 @@ def calc_second_lame(E, nu):
 _ACCUM = {"atomic": _abi.ACCUM_ATOMIC, "csr": _abi.ACCUM_CSR}
 _PATHS = {"graph": _abi.PATH_GRAPH, "kernels": _abi.PATH_KERNELS}
 _LAYOUTS = {"patch": _abi.LAYOUT_PATCH, "flat": _abi.LAYOUT_FLAT}
+_ORDERS = {"bisect": _abi.ORDER_BISECT, "morton": _abi.ORDER_MORTON, "none": _abi.ORDER_NONE,
+           True: _abi.ORDER_BISECT, False: _abi.ORDER_NONE}
 
 # Python attributes that feed the device parameter block.
 _PARAM_ATTRS = ("young_modulus", "poisson_ratio", "frame_dt", "dt", "substeps_num",
@@ -113,7 +117,10 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
             raise ValueError(f"layout must be one of {sorted(_LAYOUTS)}")
         self._layout = layout
         self._accumulate_mode = accumulate_mode
-        self._reorder = bool(reorder)
+        if reorder not in _ORDERS:
+            raise ValueError(f"reorder must be True, False or one of "
+                             f"{sorted(k for k in _ORDERS if isinstance(k, str))}")
+        self._reorder = _ORDERS[reorder]
         self._device = int(device)
         self._exec_path = exec_path
         self._scale = scale
@@ -168,7 +175,7 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         d.damping = float(self.damping)
         d.total_mass = float(self.total_mass)
         d.accumulate_mode = _ACCUM[self._accumulate_mode]
-        d.reorder = int(self._reorder)
+        d.reorder = self._reorder
         d.block_size = int(self.block_size)
         d.device = self._device
         d.layout = _LAYOUTS[self._layout]

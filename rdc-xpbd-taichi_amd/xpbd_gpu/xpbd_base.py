@@ -144,14 +144,18 @@ class XPBDSolverBase:
         self.indices = _IndexField(self)
 
     # ---- handle plumbing ----------------------------------------------------
-    def __del__(self):
+    def close(self):
+        """Frees the scene's device state now; the solver is unusable after."""
         h = getattr(self, "_h", None)
         if h:
-            try:
-                self._lib.xpbd_destroy(h)
-            except Exception:
-                pass
             self._h = None
+            self._lib.xpbd_destroy(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def _push_gravity(self):
         if self._h:

@@ -50,6 +50,30 @@ def test_errors_are_reported_without_gpu_work():
     assert "mesh" in _abi.last_error()
 
 
+@pytest.mark.skipif(not _abi.library_available(), reason="libxpbd_hip.so not built")
+def test_unknown_order_is_rejected():
+    import numpy as np
+    from xpbd_gpu.neohookean_solver import XPBDNeoHookeanSolver
+    from xpbd_gpu.sdf import PlaneSdf
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float32)
+    t = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    with pytest.raises(ValueError, match="reorder"):
+        XPBDNeoHookeanSolver((v, t), [PlaneSdf([0.0, -1.0, 0.0], [0.0, 1.0, 0.0])],
+                             reorder="hilbert")
+    lib = _abi.load()
+    d = _abi.XpbdDesc()
+    lib.xpbd_desc_init(ctypes.byref(d))
+    assert d.reorder == _abi.ORDER_BISECT
+    vf, tf = v.ravel(), t.ravel()
+    d.rest_positions = vf.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    d.tets = tf.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    d.n_verts, d.n_tets = 4, 1
+    d.reorder = 7
+    h = ctypes.c_void_p()
+    assert lib.xpbd_create(ctypes.byref(d), ctypes.byref(h)) == _abi.XPBD_EINVAL
+    assert "order" in _abi.last_error()
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_abi, "LIB_PATH", str(tmp_path / "nope.so"))
     monkeypatch.setattr(_abi, "_lib", None)

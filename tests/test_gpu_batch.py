@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 from scenes import batched_scenes, cfg1_contact, make_oracle, make_solver
-from xpbd_gpu.batch import SceneBatch, autotune
+from xpbd_gpu.batch import SceneBatch, autotune, autotune_ordering
 
 pytestmark = pytest.mark.gpu
 
@@ -76,6 +76,23 @@ def test_batch_rejects_mismatched_members():
         SceneBatch([a, b])
     with pytest.raises(ValueError):
         SceneBatch([a, a])
+
+
+def test_autotune_ordering_frees_and_picks():
+    _print_case("autotune_ordering")
+    scenes = batched_scenes(n_scenes=4, seed=5, n_range=(3, 8), iters=10)
+    built = []
+
+    def build(order):
+        solvers = [make_solver(s, reorder=order) for s in scenes]
+        built.append(solvers)
+        return solvers
+
+    best, res = autotune_ordering(build, substeps=1)
+    assert best in ("bisect", "morton", "none") and [o for o, _ in res] == ["bisect", "morton",
+                                                                           "none"]
+    assert all(t > 0 for _, t in res)
+    assert all(s._h is None for solvers in built for s in solvers)  # freed after timing
 
 
 def test_autotune_restores_state():

@@ -201,6 +201,17 @@ def test_unreordered_matches_oracle_closely():
     check_positions(scene, solver, orc, tol=1e-6)
 
 
+@pytest.mark.parametrize("order", ["bisect", "morton", "none"])
+def test_tet_orders_match_oracle(order):
+    """Every internal ordering the batch autotuner may pick meets the
+    position and residual-curve gates."""
+    _print_case("tet_order", order=order)
+    scene = cfg1_contact(n=8, frames=3)
+    solver, orc = run_pair(scene, accumulate_mode="csr", layout="patch", reorder=order)
+    check_positions(scene, solver, orc)
+    check_residual_curve(solver, orc)
+
+
 def test_patch_and_flat_agree():
     """The patch layout recomputes inv_Dm/V0 from rest positions bit-exactly, so
     both layouts in deterministic mode differ only by summation order."""
