@@ -4,9 +4,9 @@ Same flags and defaults (run_neohookean_usd.py:13-75), the same scene (a fixed
 plane at y=-2 and a fixed sphere of radius 0.3 at (0,-1,0), the mesh offset by
 (0,1,0), :82-85), the same output folders (output/<model>/xpbd_<method>_young_
 <E>/, usd_export.py:10-34) and the same console report and convergence files.
-USD export needs pxr, which this environment does not have: the USD flags are
-accepted and the run reports export as off.  `--arch cpu` is refused: this
-solver runs on the GPU only.
+The USD animation is written by this package's `usd_export` (text .usda, since
+the `pxr` library is not available; a binary .usdc request is written as
+.usda).  `--arch cpu` is refused: this solver runs on the GPU only.
 
 Example (from rdc-xpbd-taichi_amd/):
   python run_neohookean.py --model assets/tetmesh/cow.node --frames 30 \\
@@ -18,7 +18,6 @@ import argparse
 import os
 import sys
 import time
-from pathlib import Path
 
 import numpy as np
 
@@ -68,23 +67,14 @@ def build_parser() -> argparse.ArgumentParser:
     return p
 
 
-def method_tag(block_neohookean: bool, dynamic_cheby: bool) -> str:
-    model = "block_neohookean" if block_neohookean else "split_neohookean"
-    accel = "dynamic_cheby" if dynamic_cheby else "no_cheby"
-    return f"{model}_{accel}"
-
-
-def young_tag(young: float) -> str:
-    return f"{young:.3e}".replace("+", "").replace(".", "p")
+from usd_export import method_tag, young_tag  # noqa: E402  (usd_export.py:10-17)
 
 
 def build_output_paths(model_path, block_neohookean, dynamic_cheby, young, root=None):
-    name = Path(model_path).name
-    prefix = (name.split(".")[0] if "." in name else Path(name).stem) or "model"
-    out = os.path.join(root or os.getcwd(), "output", prefix)
-    run_name = f"xpbd_{method_tag(block_neohookean, dynamic_cheby)}_young_{young_tag(young)}"
-    run_dir = os.path.join(out, run_name)
-    os.makedirs(run_dir, exist_ok=True)
+    """(output_dir, run_output_dir) of usd_export.build_output_paths."""
+    import usd_export
+    out, run_dir, _ = usd_export.build_output_paths(model_path, block_neohookean, dynamic_cheby,
+                                                    young, root=root)
     return out, run_dir
 
 
@@ -110,10 +100,17 @@ def main(argv=None) -> int:
         print("run_neohookean.py: this solver runs on the GPU only (--arch gpu)", file=sys.stderr)
         return 2
     import convergence_report
+    import usd_export
 
-    solver, _ = make_solver(args)
-    _, run_dir = build_output_paths(args.model, args.block_neohookean, args.cheb, args.young,
-                                    args.output_root)
+    solver, sdf = make_solver(args)
+    _, run_dir, usd_path = usd_export.build_output_paths(
+        args.model, args.block_neohookean, args.cheb, args.young, use_binary=args.use_binary,
+        root=args.output_root)
+    renderer = usd_export.create_renderer(args, usd_path, args.frames, args.fps)
+    if renderer is not None:
+        usd_path = renderer.filepath
+    usd_export.add_solver_mesh(renderer, solver)
+    usd_export.add_sdfs(renderer, sdf)
     plot_dir = None
     if args.monitor_convergence and args.plot_frames:
         plot_dir = convergence_report.setup_convergence_callback(
@@ -129,10 +126,10 @@ def main(argv=None) -> int:
     print(f"Dynamic Chebyshev: {'ON' if args.cheb else 'OFF'}")
     print(f"Frames: {args.frames}, substeps: {args.substeps}, iterations: {args.iterations}")
     print(f"Young's modulus: {args.young:.2e}, Poisson ratio: {args.poisson}")
-    print("USD export: OFF (pxr is not available)")
+    print(f"USD export: {'OFF' if renderer is None else usd_path}")
     print("=" * 70)
 
-    times = []
+    times, usd_times = [], []
     t0 = time.time()
     for frame in range(args.frames):
         if frame % 10 == 0:
@@ -142,6 +139,17 @@ def main(argv=None) -> int:
         times.append(time.time() - s)
         if args.monitor_convergence and args.print_convergence and not args.plot_frames:
             convergence_report.print_frame_convergence(solver, frame, args.substeps)
+        if renderer is not None:
+            s = time.time()
+            renderer.render()
+            usd_times.append(time.time() - s)
+    save_time = 0.0
+    if renderer is not None:
+        print("\n" + "=" * 70)
+        print("Saving USD file...")
+        s = time.time()
+        renderer.save()
+        save_time = time.time() - s
     total = time.time() - t0
     solver.check_finite()
 
@@ -152,6 +160,10 @@ def main(argv=None) -> int:
     print(f"Solver total: {np.sum(times):.3f} s")
     print(f"Solver per frame: {np.mean(times):.4f} s")
     print(f"Solver FPS: {1.0 / np.mean(times):.2f}")
+    print(f"USD save time: {save_time:.3f} s")
+    if renderer is not None and usd_times:
+        print(f"USD export total: {np.sum(usd_times):.3f} s")
+        print(f"USD file size: {renderer.get_file_size_mb():.1f} MB")
     n_iter = args.frames * args.substeps * args.iterations
     print(f"Total constraint iterations: {n_iter:,}")
     print(f"Time per iteration: {np.sum(times) / n_iter * 1e6:.2f} us")
@@ -160,6 +172,8 @@ def main(argv=None) -> int:
                                                   export_csv=args.export_csv)
     print("\nOUTPUTS")
     print("=" * 70)
+    if renderer is not None:
+        print(f"USD animation: {usd_path}")
     if plot_dir is not None:
         print(f"Convergence plots: {plot_dir}")
     print("=" * 70)

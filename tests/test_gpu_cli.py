@@ -30,3 +30,8 @@ def test_cli_run_writes_the_reference_outputs(tmp_path, capsys):
     assert "f0000_s00.json" in plots and "f0000_s00.png" in plots
     doc = json.load(open(run / "convergence_plots" / "f0001_s00.json"))
     assert len(doc["hydro_errors"]) == 10 and doc["solver_type"] == "XPBDNeoHookeanSolver"
+    # the animation: --use-binary (default) asks for .usdc, written as the .usda text stage
+    usd = tmp_path / "output" / "box" / "xpbd_block_neohookean_dynamic_cheby_young_1p000e06.usda"
+    text = usd.read_text()
+    assert "points.timeSamples" in text and "\n            2: [" in text
+    assert 'def Sphere "sdf_sphere_1"' in text and 'def Mesh "sdf_plane_0"' in text
