@@ -28,8 +28,6 @@ hipError_t launch_patch_residual(const SceneDev* scenes, const PatchItem* items,
 hipError_t patch_kernel_occupancy(int* project_blocks_per_cu, int* residual_blocks_per_cu);
 hipError_t launch_final_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                               int n_iter, hipStream_t st);
-hipError_t launch_patch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
-                              hipStream_t st);
 hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream_t st);
 hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                         hipStream_t st);

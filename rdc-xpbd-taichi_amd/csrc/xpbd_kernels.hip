@@ -867,24 +867,6 @@ __global__ void k_blend(const SceneDev* __restrict__ scenes, const WorkItem* __r
   S.pos[v] = make_float4(nw.x, nw.y, nw.z, y4.w);
 }
 
-// Chebyshev blend with x_J formed from the patch slots (patch layout).
-__global__ void k_patch_blend(const SceneDev* __restrict__ scenes,
-                              const WorkItem* __restrict__ map) {
-  const WorkItem wi = map[blockIdx.x];
-  const SceneDev& S = scenes[wi.scene];
-  const int v = wi.first + threadIdx.x;
-  if (v >= S.n_verts) return;
-  const float omega = float(S.ctrl[0].omega);  // both slots hold the update of this iteration
-  const float gamma = S.gamma;
-  const float4 xj4 = jacobi_average_slots(S, v, 0);
-  const float4 y4 = S.pos[v];
-  const float3 y = f3(y4), xj = f3(xj4), pr = f3(S.prev[v]);
-  const float3 x_new = gamma * (xj - y) + y;
-  const float3 nw = pr + omega * (x_new - pr);
-  S.prev[v] = make_float4(y.x, y.y, y.z, 0.f);
-  S.pos[v] = make_float4(nw.x, nw.y, nw.z, y4.w);
-}
-
 // Blend of the last iteration of the fused sequence: y^n_iter into pos.
 __global__ void k_final_blend(const SceneDev* __restrict__ scenes,
                               const WorkItem* __restrict__ map, int n_iter) {
@@ -1004,11 +986,6 @@ hipError_t launch_reduce(const SceneDev* scenes, int n_scenes, int it, hipStream
 hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                         hipStream_t st) {
   k_blend<<<nblocks, bs, 0, st>>>(scenes, vmap);
-  return hipGetLastError();
-}
-hipError_t launch_patch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
-                              hipStream_t st) {
-  k_patch_blend<<<nblocks, bs, 0, st>>>(scenes, vmap);
   return hipGetLastError();
 }
 hipError_t launch_final_blend(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
