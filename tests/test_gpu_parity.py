@@ -212,6 +212,43 @@ def test_tet_orders_match_oracle(order):
     check_residual_curve(solver, orc)
 
 
+@pytest.mark.parametrize("order", ["bisect", "none"])
+def test_high_valence_vertex_matches_oracle(order):
+    """fan_mesh: one vertex in 600 tets, split over several patches by the
+    255-corner rank limit; stretched by 10 % and relaxed without gravity."""
+    from scenes import Scene, base_params, fan_mesh, plane
+    _print_case("fan", order=order)
+    m = fan_mesh()
+    sc = Scene("fan", m, [plane(-10.0)],
+               base_params(young_modulus=1e5, poisson_ratio=0.3, substeps_num=2,
+                           constraints_iter=20),
+               total_mass=1.0, gravity=(0.0, 0.0, 0.0), frames=2)
+    sc.init_x = (m.verts * 1.1).astype(np.float32)
+    solver, orc = run_pair(sc, accumulate_mode="csr", layout="patch", reorder=order)
+    check_positions(sc, solver, orc)
+    check_residual_curve(solver, orc)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_isolated_vertices_match_oracle(layout):
+    """Vertices no tet references get no mass (inv_m = 0, so they never move)
+    and turn off the fused Chebyshev sequence; the rest of the mesh still
+    matches the oracle."""
+    from scenes import Scene
+    from xpbd_gpu.mesh import TetMesh
+    _print_case("isolated", layout=layout)
+    base = cfg1_contact(n=4, frames=3)
+    extra = np.array([[3.0, 1.0, 0.0], [3.5, 1.0, 0.0], [-2.0, 0.5, 1.0]], dtype=np.float32)
+    m = TetMesh(np.vstack([base.mesh.verts, extra]), base.mesh.tets)
+    sc = Scene("isolated", m, base.sdfs, base.params, total_mass=base.total_mass,
+               frames=base.frames)
+    solver, orc = run_pair(sc, accumulate_mode="csr", layout=layout)
+    x = solver.mesh.verts.x.to_numpy()
+    assert np.array_equal(x[-3:], extra)
+    check_positions(sc, solver, orc, tol=1e-5)
+    check_residual_curve(solver, orc)
+
+
 def test_patch_and_flat_agree():
     """The patch layout recomputes inv_Dm/V0 from rest positions bit-exactly, so
     both layouts in deterministic mode differ only by summation order."""

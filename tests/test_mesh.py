@@ -86,6 +86,20 @@ def test_patch_builder_invariants():
         _abi.patch_stats(np.array([[0, 1, 2, 9]]), 4)
 
 
+def test_patch_builder_splits_at_255_corners():
+    """A vertex in 600 tets: corner ranks are bytes, so no patch may hold more
+    than 255 of its tets, greedy or bisected."""
+    from scenes import fan_mesh
+    from xpbd_gpu import _abi
+    if not _abi.library_available():
+        pytest.skip("libxpbd_hip.so not built")
+    m = fan_mesh()
+    assert (m.tets == 0).any(axis=1).all()
+    for pos in (None, m.verts):
+        st = _abi.patch_stats(m.tets, m.n_verts, pos)
+        assert st["max_patch_tets"] <= 255 and st["n_patches"] >= 3
+
+
 def test_bisection_patches_are_compact():
     """Coordinate bisection stages far fewer vertices per tet than greedy cuts
     of the given order."""
