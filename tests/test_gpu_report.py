@@ -15,8 +15,11 @@ import os
 import numpy as np
 import pytest
 
-from scenes import cfg1_contact, cfg1_plain_small, cfg2_plain_large, cfg3_stress, make_oracle
+from scenes import (batched_scenes, cfg1_contact, cfg1_plain_small, cfg2_plain_large, cfg3_stress,
+                    make_oracle)
 from test_gpu_parity import POS_TOL, check_residual_curve, make_solver
+from xpbd_gpu.batch import SceneBatch
+from xpbd_gpu.serve import Request, Server, served_scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -118,3 +121,63 @@ def test_report_cfg3_stress():
     e["det_f_after"] = histogram(after)
     e["det_f_after_min_max"] = [float(after.min()), float(after.max())]
     _write("cfg3_stress", e)
+
+
+def _sample_errors(scenes, xs, frames):
+    """Per scene: the device positions xs[k] after frames[k] frames against
+    the f32 and f64 oracles; returns the worst of each figure and gates them."""
+    worst = {"gpu_vs_f64_x_diag": 0.0, "f32_oracle_vs_f64_x_diag": 0.0,
+             "gpu_vs_f32_oracle_x_diag": 0.0}
+    for sc, x, f in zip(scenes, xs, frames):
+        o32 = make_oracle(sc, threads=THREADS)
+        o64 = make_oracle(sc, precision=64, threads=THREADS)
+        o32.frame(f)
+        o64.frame(f)
+        x = np.asarray(x, np.float64)
+        e_gpu = float(np.abs(x - o64.x).max() / sc.diag)
+        e_f32 = float(np.abs(o32.x - o64.x).max() / sc.diag)
+        e_pair = float(np.abs(x - o32.x).max() / sc.diag)
+        print(f"  {sc.name}: {sc.mesh.n_tets} tets, {f} frames: vs f64 {e_gpu:.2e}, "
+              f"f32 oracle vs f64 {e_f32:.2e}, vs f32 oracle {e_pair:.2e}", flush=True)
+        assert np.all(np.isfinite(x))
+        assert e_pair <= POS_TOL or e_gpu <= 2.0 * e_f32
+        for key, v in (("gpu_vs_f64_x_diag", e_gpu), ("f32_oracle_vs_f64_x_diag", e_f32),
+                       ("gpu_vs_f32_oracle_x_diag", e_pair)):
+            worst[key] = max(worst[key], v)
+    return worst
+
+
+def test_report_batched_sample():
+    """Eight scenes of the batched configuration (every 32nd of the 256),
+    advanced together by one batch for a frame."""
+    print("[case] report batched_sample", flush=True)
+    scenes = batched_scenes(n_scenes=256)[::32]
+    solvers = [make_solver(sc) for sc in scenes]
+    with SceneBatch(solvers) as b:
+        b.frame(1)
+        b.synchronize()
+    worst = _sample_errors(scenes, [s.mesh.verts.x.to_numpy() for s in solvers],
+                           [1] * len(scenes))
+    e = {"scenes": len(scenes), "tets": int(sum(sc.mesh.n_tets for sc in scenes)),
+         "run": "1 frame in one batch", "iterations": int(scenes[0].params["constraints_iter"]),
+         **{f"worst_{k}": v for k, v in worst.items()}}
+    _write("batched_sample", e)
+
+
+def test_report_served_sample():
+    """Four scenes of the served configuration, advanced by overlapping
+    requests of 1-2 frames through the serving loop."""
+    print("[case] report served_sample", flush=True)
+    specs = served_scenes(n_scenes=64)
+    pick = sorted(range(len(specs)), key=lambda k: specs[k].mesh.n_tets)[20:60:10]
+    reqs = [Request(i, k, 1 + i % 2, 0.002 * i) for i, k in enumerate(pick * 2)]
+    srv = Server(specs)
+    srv.run(reqs)
+    frames = [sum(r.frames for r in reqs if r.scene == k) for k in pick]
+    xs = [srv.resident[k].mesh.verts.x.to_numpy() for k in pick]
+    worst = _sample_errors([specs[k] for k in pick], xs, frames)
+    e = {"scenes": len(pick), "tets": int(sum(specs[k].mesh.n_tets for k in pick)),
+         "run": f"{min(frames)}-{max(frames)} frames per scene in {len(reqs)} overlapping requests",
+         "iterations": int(specs[0].params["constraints_iter"]),
+         **{f"worst_{k}": v for k, v in worst.items()}}
+    _write("served_sample", e)
