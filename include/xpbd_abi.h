@@ -243,6 +243,11 @@ int xpbd_batch_opts_init(xpbd_batch_opts* o);
 int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_batch_opts* opts,
                       xpbd_batch** out);
 int xpbd_batch_destroy(xpbd_batch* b);
+/* Re-targets the batch at another member list (same rules as create), reusing
+ * its stream and, where large enough, its device buffers; waits for the
+ * batch's queued work.  For serving loops whose membership changes at
+ * substep boundaries. */
+int xpbd_batch_set_members(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_scenes);
 /* one substep (sub_dt = dt / substeps_num of each scene) of every member */
 int xpbd_batch_substep(xpbd_batch* b);
 /* n_frames frames of every member (solve(), neohookean_solver.py:562-587) */

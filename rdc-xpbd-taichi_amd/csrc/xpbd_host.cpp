@@ -65,16 +65,27 @@ int set_error(int code, const char* fmt, ...) {
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
-  size_t n = 0;
+  size_t n = 0;    // elements in use
+  size_t cap = 0;  // elements allocated
   int alloc(size_t count) {
     release();
-    n = count;
+    n = cap = count;
     if (count == 0) return XPBD_OK;
     HIP_TRY(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
     return XPBD_OK;
   }
   int upload(const T* src, size_t count) {
     TRY(alloc(count));
+    if (count) HIP_TRY(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    return XPBD_OK;
+  }
+  // upload into the existing allocation when it is large enough
+  int upload_reuse(const T* src, size_t count) {
+    if (count > cap || !p) {
+      TRY(alloc(count));
+    } else {
+      n = count;
+    }
     if (count) HIP_TRY(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
     return XPBD_OK;
   }
@@ -85,7 +96,7 @@ struct DevBuf {
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
-    n = 0;
+    n = cap = 0;
   }
   ~DevBuf() { release(); }
 };
@@ -1662,6 +1673,79 @@ int batch_sync_table(xpbd_batch* b) {
   return XPBD_OK;
 }
 
+// Members of one batch: distinct, on the batch's device, patch layout, and
+// the same iteration count, time step and mode flags.  Their own streams are
+// drained, since from now on they run on the batch's stream.
+int batch_check_members(xpbd_solver* const* scenes, int n, int device) {
+  xpbd_solver* s0 = scenes[0];
+  if (!s0) return set_error(XPBD_EINVAL, "NULL scene");
+  for (int i = 0; i < n; ++i) {
+    xpbd_solver* s = scenes[i];
+    if (!s) return set_error(XPBD_EINVAL, "NULL scene %d", i);
+    for (int j = 0; j < i; ++j)
+      if (scenes[j] == s) return set_error(XPBD_EINVAL, "scene %d appears twice", i);
+    if (s->device != device) return set_error(XPBD_EINVAL, "scene %d on another device", i);
+    if (s->layout != XPBD_LAYOUT_PATCH)
+      return set_error(XPBD_EINVAL, "scene %d: batches need the patch layout", i);
+    const xpbd_desc &a = s->prm, &r = s0->prm;
+    if (a.constraints_iter != r.constraints_iter || a.substeps_num != r.substeps_num ||
+        a.frame_dt != r.frame_dt || a.dt != r.dt || a.cheb_enable != r.cheb_enable ||
+        a.block_neohookean_enable != r.block_neohookean_enable ||
+        a.monitor_convergence != r.monitor_convergence || a.accumulate_mode != r.accumulate_mode ||
+        s->has_isolated != s0->has_isolated)
+      return set_error(XPBD_EINVAL, "scene %d: iteration count, time step or mode flags differ", i);
+  }
+  for (int i = 0; i < n; ++i) HIP_TRY(hipStreamSynchronize(scenes[i]->stream));
+  return XPBD_OK;
+}
+
+// The batch's concatenated work lists (vertex blocks and patch items in member
+// order), its persistent grids and ranges, and its scene table, for the given
+// members; device buffers large enough from an earlier membership are reused.
+int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes) {
+  const xpbd_batch_opts& o = b->opts;
+  b->scenes.assign(scenes, scenes + n_scenes);
+  b->n_verts = b->n_tets = 0;
+  std::vector<WorkItem> vm;
+  std::vector<PatchItem> items;
+  for (int i = 0; i < n_scenes; ++i) {
+    xpbd_solver* s = b->scenes[i];
+    for (int v = 0; v < s->nv; v += o.vertex_block) vm.push_back({i, v});
+    for (PatchItem it : s->pitems_host) {
+      it.scene = i;
+      items.push_back(it);
+    }
+    b->n_verts += s->nv;
+    b->n_tets += s->nt;
+  }
+  const int nx = o.xcd_split ? xcd_count() : 1;
+  std::vector<WorkItem> vmo(vm.size());
+  const std::vector<int> order = xcd_order(int(vm.size()), nx);
+  for (size_t k = 0; k < vm.size(); ++k) vmo[k] = vm[order[k]];
+  b->nvb = int(vmo.size());
+  b->n_items = int(items.size());
+  TRY(b->vmap.upload_reuse(vmo.data(), vmo.size()));
+  TRY(b->pitems.upload_reuse(items.data(), items.size()));
+  int occ_p = 0, occ_r = 0, n_cu = 0;
+  HIP_TRY(xpbd::patch_kernel_occupancy(&occ_p, &occ_r));
+  HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
+  const long long full_p = (long long)n_cu * std::max(1, occ_p);
+  const long long full_r = (long long)n_cu * std::max(1, occ_r);
+  b->grid_proj = int(std::max(1LL, std::min<long long>(b->n_items, full_p * o.grid_pct / 100)));
+  b->grid_res = int(std::max(1LL, std::min<long long>(b->n_items, full_r * o.grid_pct / 100)));
+  std::vector<int2> rp, rr;
+  TRY(xcd_ranges(b->n_items, b->grid_proj, nx, rp));
+  TRY(grouped_ranges(items, b->grid_res, nx, rr, &b->grid_res));
+  TRY(b->prange_proj.upload_reuse(rp.data(), rp.size()));
+  TRY(b->prange_res.upload_reuse(rr.data(), rr.size()));
+  if (b->table.cap < size_t(n_scenes)) TRY(b->table.alloc(n_scenes));
+  b->table.n = size_t(n_scenes);
+  b->table_host.clear();  // forces the upload
+  TRY(batch_sync_table(b));
+  b->proto = b->table_host[0];
+  return XPBD_OK;
+}
+
 // One substep of every member: predict + collide + save, the Jacobi
 // iterations, the final blend and advance (the fused patch sequence of
 // enqueue_iterations, over the concatenated work list).
@@ -1776,66 +1860,28 @@ int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_b
   if (o.vertex_block < 64 || o.vertex_block > 1024 || o.vertex_block % 64)
     return set_error(XPBD_EINVAL, "vertex_block must be a multiple of 64 in [64, 1024]");
   if (o.grid_pct < 1 || o.grid_pct > 100) return set_error(XPBD_EINVAL, "grid_pct in [1, 100]");
-  xpbd_solver* s0 = scenes[0];
-  if (!s0) return set_error(XPBD_EINVAL, "NULL scene");
-  b->device = s0->device;
+  if (!scenes[0]) return set_error(XPBD_EINVAL, "NULL scene");
+  b->device = scenes[0]->device;
   HIP_TRY(hipSetDevice(b->device));
-  for (int i = 0; i < n_scenes; ++i) {
-    xpbd_solver* s = scenes[i];
-    if (!s) return set_error(XPBD_EINVAL, "NULL scene %d", i);
-    for (int j = 0; j < i; ++j)
-      if (scenes[j] == s) return set_error(XPBD_EINVAL, "scene %d appears twice", i);
-    if (s->device != b->device) return set_error(XPBD_EINVAL, "scene %d on another device", i);
-    if (s->layout != XPBD_LAYOUT_PATCH) return set_error(XPBD_EINVAL, "scene %d: batches need the patch layout", i);
-    const xpbd_desc &a = s->prm, &r = s0->prm;
-    if (a.constraints_iter != r.constraints_iter || a.substeps_num != r.substeps_num ||
-        a.frame_dt != r.frame_dt || a.dt != r.dt || a.cheb_enable != r.cheb_enable ||
-        a.block_neohookean_enable != r.block_neohookean_enable ||
-        a.monitor_convergence != r.monitor_convergence || a.accumulate_mode != r.accumulate_mode ||
-        s->has_isolated != s0->has_isolated)
-      return set_error(XPBD_EINVAL, "scene %d: iteration count, time step or mode flags differ", i);
-    HIP_TRY(hipStreamSynchronize(s->stream));
-    b->scenes.push_back(s);
-  }
+  TRY(batch_check_members(scenes, n_scenes, b->device));
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  // concatenated work lists, in member order
-  std::vector<WorkItem> vm;
-  std::vector<PatchItem> items;
-  for (int i = 0; i < n_scenes; ++i) {
-    xpbd_solver* s = b->scenes[i];
-    for (int v = 0; v < s->nv; v += o.vertex_block) vm.push_back({i, v});
-    for (PatchItem it : s->pitems_host) {
-      it.scene = i;
-      items.push_back(it);
-    }
-    b->n_verts += s->nv;
-    b->n_tets += s->nt;
-  }
-  const int nx = o.xcd_split ? xcd_count() : 1;
-  std::vector<WorkItem> vmo(vm.size());
-  const std::vector<int> order = xcd_order(int(vm.size()), nx);
-  for (size_t k = 0; k < vm.size(); ++k) vmo[k] = vm[order[k]];
-  b->nvb = int(vmo.size());
-  b->n_items = int(items.size());
-  TRY(b->vmap.upload(vmo.data(), vmo.size()));
-  TRY(b->pitems.upload(items.data(), items.size()));
-  int occ_p = 0, occ_r = 0, n_cu = 0;
-  HIP_TRY(xpbd::patch_kernel_occupancy(&occ_p, &occ_r));
-  HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
-  const long long full_p = (long long)n_cu * std::max(1, occ_p);
-  const long long full_r = (long long)n_cu * std::max(1, occ_r);
-  b->grid_proj = int(std::max(1LL, std::min<long long>(b->n_items, full_p * o.grid_pct / 100)));
-  b->grid_res = int(std::max(1LL, std::min<long long>(b->n_items, full_r * o.grid_pct / 100)));
-  std::vector<int2> rp, rr;
-  TRY(xcd_ranges(b->n_items, b->grid_proj, nx, rp));
-  TRY(grouped_ranges(items, b->grid_res, nx, rr, &b->grid_res));
-  TRY(b->prange_proj.upload(rp.data(), rp.size()));
-  TRY(b->prange_res.upload(rr.data(), rr.size()));
-  TRY(b->table.alloc(n_scenes));
-  TRY(batch_sync_table(b.get()));
-  b->proto = b->table_host[0];
+  TRY(batch_build(b.get(), scenes, n_scenes));
   *out = b.release();
   return XPBD_OK;
+}
+
+int xpbd_batch_set_members(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_scenes) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  if (!scenes || n_scenes <= 0) return set_error(XPBD_EINVAL, "batch needs scenes");
+  HIP_TRY(hipSetDevice(b->device));
+  TRY(batch_check_members(scenes, n_scenes, b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));  // the lists below are in use until then
+  if (b->gexec) {
+    HIP_TRY(hipGraphExecDestroy(b->gexec));
+    b->gexec = nullptr;
+    b->graph_substeps = 0;
+  }
+  return batch_build(b, scenes, n_scenes);
 }
 
 int xpbd_batch_destroy(xpbd_batch* b) {

@@ -52,6 +52,9 @@ class SceneBatch:
         _abi.check(self._lib.xpbd_batch_create(arr, len(self.solvers), ctypes.byref(o),
                                                ctypes.byref(h)))
         self._h = h.value
+        self._refresh_info()
+
+    def _refresh_info(self):
         ns = ctypes.c_int32()
         nv = ctypes.c_int64()
         nt = ctypes.c_int64()
@@ -63,6 +66,20 @@ class SceneBatch:
         s0 = self.solvers[0]
         self.iters = int(s0.constraints_iter)
         self.substeps_num = int(s0.substeps_num)
+
+    def set_members(self, solvers):
+        """Re-targets the batch at another list of scenes (xpbd_batch_set_members):
+        the stream and large-enough device buffers are kept; queued work is
+        waited for."""
+        solvers = list(solvers)
+        if not solvers:
+            raise ValueError("a batch needs at least one scene")
+        for s in solvers:
+            s._sync_params()
+        arr = (ctypes.c_void_p * len(solvers))(*[s._h for s in solvers])
+        _abi.check(self._lib.xpbd_batch_set_members(self._h, arr, len(solvers)))
+        self.solvers = solvers
+        self._refresh_info()
 
     def close(self):
         if getattr(self, "_h", None):

@@ -180,9 +180,11 @@ class Server:
             k = tuple(sorted(active))
             if k != key:
                 tb = time.perf_counter()
-                if batch is not None:
-                    batch.close()
-                batch = SceneBatch([self.resident[j] for j in k], **self.batch_opts)
+                members = [self.resident[j] for j in k]
+                if batch is None:
+                    batch = SceneBatch(members, **self.batch_opts)
+                else:  # same stream and buffers, new member list
+                    batch.set_members(members)
                 key = k
                 self.rebuilds += 1
                 self.build_s += time.perf_counter() - tb

@@ -67,6 +67,36 @@ def test_batch_substeps_and_mixed_sizes():
         assert np.array_equal(a.mesh.verts.x.to_numpy(), m.mesh.verts.x.to_numpy())
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_set_members_matches_fresh_batches(graph):
+    """A batch re-targeted at other members (growing, shrinking and reordering
+    its list) steps each scene exactly as plain solves do."""
+    _print_case("set_members", graph=graph)
+    scenes = batched_scenes(n_scenes=5, seed=11, n_range=(3, 10), iters=15)
+    alone = [make_solver(s) for s in scenes]
+    mixed = [make_solver(s) for s in scenes]
+    plan = [[0, 1], [1, 2, 3, 4], [4], [3, 0, 2]]  # members per substep round
+    done = [0] * len(scenes)
+    with SceneBatch([mixed[0]], graph=graph) as b:
+        for members in plan:
+            b.set_members([mixed[k] for k in members])
+            assert b.n_scenes == len(members)
+            b.substep(2)
+            b.synchronize()
+            for k in members:
+                done[k] += 2
+    for k, n in enumerate(done):
+        sub_dt = alone[k].dt / alone[k].substeps_num
+        for _ in range(n):
+            alone[k].apply_external_forces(sub_dt)
+            alone[k].solve_step(sub_dt)
+            alone[k].advance(sub_dt, alone[k].damping)
+        assert np.array_equal(alone[k].mesh.verts.x.to_numpy(), mixed[k].mesh.verts.x.to_numpy()), k
+    with pytest.raises(ValueError):
+        b2 = SceneBatch([mixed[0]])
+        b2.set_members([mixed[1], mixed[1]])
+
+
 def test_batch_rejects_mismatched_members():
     a = make_solver(cfg1_contact(n=2))
     sc = cfg1_contact(n=2)
