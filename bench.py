@@ -27,7 +27,7 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 HBM_PEAK_GBPS = 8000.0
-PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_patch_project_cfg2_pmc.txt")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_patch_project_jittered_box64_pmc.txt")
 PMC_COMMANDS = """\
 cd <repo root>; export TMPDIR=/tmp; D=<output folder>
 rocprofv3 --pmc FETCH_SIZE --output-format csv -d $D/fetch -o run -- python3 bench.py --steps 1 --warmup 0 --no-cpu-baseline --roofline-frames 0 --plain-only
@@ -63,7 +63,7 @@ def _pmc_per_kernel(path: str) -> dict:
 
 
 def summarize_pmc(folder: str, copy_mib: int = 512) -> str:
-    """Per-dispatch HBM-side traffic of every kernel of one cfg2 frame from the
+    """Per-dispatch HBM-side traffic of every kernel of one frame of the 64^3 jittered box, from the
     FETCH_SIZE and WRITE_SIZE passes, each scaled by what the same counter
     reads for a plain device copy of known size (the copy's own kernel)."""
     f = _pmc_per_kernel(os.path.join(folder, "fetch", "run_counter_collection.csv"))
@@ -75,7 +75,7 @@ def summarize_pmc(folder: str, copy_mib: int = 512) -> str:
     f_read = float(np.mean(cf[ck]))
     w_read = float(np.mean(cw[ck]))
     f_scale, w_scale = copy_kib / f_read, copy_kib / w_read
-    lines = ["# HBM-side traffic per dispatch of the cfg2 frame (64^3 jittered box, 1 572 864 tets,",
+    lines = ["# HBM-side traffic per dispatch of a 64^3 jittered-box frame (1 572 864 tets,",
              "# 274 625 vertices, 10 substeps x 100 iterations), rocprofv3 FETCH_SIZE / WRITE_SIZE.",
              "# Made by:"]
     lines += ["#   " + c for c in PMC_COMMANDS.splitlines()]
@@ -115,7 +115,7 @@ def summarize_stats(folder: str) -> str:
     with open(os.path.join(folder, "run_kernel_trace.csv")) as fh:
         proj = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]))
                       for r in csv.DictReader(fh) if "k_patch_project" in r["Kernel_Name"])
-    lines = ["# rocprofv3 kernel statistics of the cfg2 bench (64^3 jittered box, 1 572 864 tets,",
+    lines = ["# rocprofv3 kernel statistics of the 64^3 jittered-box bench (1 572 864 tets,",
              "# 274 625 vertices, 10 substeps x 100 iterations per frame): 1 warmup + 3 timed",
              "# graph frames, then the event-timed frames (8 projection replays per iteration,",
              "# then one event pair per projection).",
@@ -348,13 +348,13 @@ def main():
               "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
-    from xpbd_gpu.scenes import cfg2_plain_large, make_solver
+    from xpbd_gpu.scenes import jittered_box, make_solver
 
-    scene = cfg2_plain_large(n=args.n)
+    scene = jittered_box(n=args.n)
     T, V = scene.mesh.n_tets, scene.mesh.n_verts
     iters = scene.params["constraints_iter"]
     subs = scene.params["substeps_num"]
-    print(f"[bench] workload cfg2: {V} verts, {T} tets, {subs} substeps x {iters} iterations "
+    print(f"[bench] workload: {V} verts, {T} tets, {subs} substeps x {iters} iterations "
           f"per frame; accumulate={args.accumulate} exec={args.exec_path}", file=sys.stderr,
           flush=True)
     solver = make_solver(scene, accumulate_mode=args.accumulate, exec_path=args.exec_path,
@@ -413,7 +413,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"cfg2 single large mesh: {args.n}^3 jittered box, {T} tets, {V} verts, "
+            "workload": f"single large mesh: {args.n}^3 jittered box, {T} tets, {V} verts, "
                         f"{subs} substeps x {iters} iterations per step (one frame)",
             "accumulate_mode": args.accumulate,
             "layout": args.layout,

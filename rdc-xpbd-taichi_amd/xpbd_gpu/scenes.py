@@ -59,31 +59,31 @@ def base_params(**kw):
     return p
 
 
-def cfg1_plain_small(n=16, drop=0.5, frames=10) -> Scene:
+def dropped_cube(n=16, drop=0.5, frames=10) -> Scene:
     """Unit cube, n^3 hexes x 6 tets, no jitter, dropped from `drop` onto the
     plane y=0; E=1e5, nu=0.45, density 1000; 5 substeps x 30 iterations."""
     m = box_grid(n, origin=(0.0, drop, 0.0))
-    return Scene("cfg1", m, [plane(0.0)],
+    return Scene("dropped_cube", m, [plane(0.0)],
                  base_params(young_modulus=1e5, poisson_ratio=0.45, substeps_num=5,
                              constraints_iter=30),
                  total_mass=1000.0, frames=frames)
 
 
-def cfg1_contact(n=16, frames=10) -> Scene:
-    """cfg1's cube resting on the plane from the start, so that gravity and
-    contact deform it within the 10 frames (cfg1 as stated free-falls and never
+def resting_cube(n=16, frames=10) -> Scene:
+    """The dropped cube resting on the plane from the start, so that gravity and
+    contact deform it within the 10 frames (dropped from 0.5 it free-falls and never
     reaches the plane)."""
-    s = cfg1_plain_small(n=n, drop=0.0, frames=frames)
-    s.name = "cfg1_contact"
+    s = dropped_cube(n=n, drop=0.0, frames=frames)
+    s.name = "resting_cube"
     return s
 
 
-def cfg2_plain_large(n=64, seed=0, frames=10, substeps_only=None, iters=100) -> Scene:
+def jittered_box(n=64, seed=0, frames=10, substeps_only=None, iters=100) -> Scene:
     """[0,1]^3 box, n^3 hexes x 6 tets, jitter +-0.2 cell (seed), E=1e6,
     nu=0.48, density 1000, resting on the plane y=0 with a fixed sphere r=0.3
     centred on the bottom face; 10 substeps x 100 iterations."""
     m = box_grid(n, jitter=0.2, seed=seed)
-    return Scene("cfg2", m, [plane(0.0), sphere((0.5, 0.0, 0.5), 0.3)],
+    return Scene("jittered_box", m, [plane(0.0), sphere((0.5, 0.0, 0.5), 0.3)],
                  base_params(young_modulus=1e6, poisson_ratio=0.48, substeps_num=10,
                              constraints_iter=iters),
                  total_mass=1000.0, frames=frames, substeps_only=substeps_only)
@@ -111,13 +111,13 @@ def twist_stretch(verts: np.ndarray, seed: int) -> np.ndarray:
     return (out + c).astype(np.float32)
 
 
-def cfg3_stress(n=32, seed=0, iters=200) -> Scene:
+def twisted_grid(n=32, seed=0, iters=200) -> Scene:
     """Unjittered n^3 grid, initial x from a seeded affine stretch plus a
     180-degree twist about z, zero velocity, no gravity; one substep of 200
     iterations (frame_dt = dt, substeps 1).  E=1e6, nu=0.48; a plane far below
     keeps the reference's required SDF inactive."""
     m = box_grid(n)
-    s = Scene("cfg3", m, [plane(-10.0)],
+    s = Scene("twisted_grid", m, [plane(-10.0)],
               base_params(young_modulus=1e6, poisson_ratio=0.48, substeps_num=1,
                           constraints_iter=iters),
               total_mass=1000.0, gravity=(0.0, 0.0, 0.0), frames=1)

@@ -3,7 +3,7 @@ scenes alone (bit-exact, deterministic mode) and against the oracle."""
 import numpy as np
 import pytest
 
-from scenes import batched_scenes, cfg1_contact, make_oracle, make_solver
+from scenes import batched_scenes, resting_cube, make_oracle, make_solver
 from xpbd_gpu.batch import SceneBatch, autotune, autotune_ordering
 
 pytestmark = pytest.mark.gpu
@@ -49,7 +49,7 @@ def test_batch_member_matches_oracle():
 
 def test_batch_substeps_and_mixed_sizes():
     _print_case("batch_substeps")
-    scenes = batched_scenes(n_scenes=3, seed=7, n_range=(2, 12), iters=10) + [cfg1_contact(n=4)]
+    scenes = batched_scenes(n_scenes=3, seed=7, n_range=(2, 12), iters=10) + [resting_cube(n=4)]
     scenes[-1].params.update(substeps_num=5, constraints_iter=10)
     solvers = [make_solver(s) for s in scenes]
     ref = [make_solver(s) for s in scenes]
@@ -98,8 +98,8 @@ def test_set_members_matches_fresh_batches(graph):
 
 
 def test_batch_rejects_mismatched_members():
-    a = make_solver(cfg1_contact(n=2))
-    sc = cfg1_contact(n=2)
+    a = make_solver(resting_cube(n=2))
+    sc = resting_cube(n=2)
     sc.params["constraints_iter"] = 7
     b = make_solver(sc)
     with pytest.raises(ValueError):

@@ -13,7 +13,7 @@ Gates (stated per test):
 import numpy as np
 import pytest
 
-from scenes import (cfg1_contact, cfg1_plain_small, cfg2_plain_large, cfg3_stress, make_oracle,
+from scenes import (resting_cube, dropped_cube, jittered_box, twisted_grid, make_oracle,
                     make_solver)
 from xpbd_gpu.mesh import box_grid
 
@@ -91,7 +91,7 @@ LAYOUTS = ["patch", "flat"]
 @pytest.mark.parametrize("mode", ["atomic", "csr"])
 def test_small_box_contact(mode, layout):
     _print_case("small_box_contact", mode=mode, layout=layout)
-    scene = cfg1_contact(n=4, frames=5)
+    scene = resting_cube(n=4, frames=5)
     solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
     check_positions(scene, solver, orc, tol=1e-5)
     check_residual_curve(solver, orc)
@@ -99,19 +99,19 @@ def test_small_box_contact(mode, layout):
 
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("mode", ["atomic", "csr"])
-def test_cfg1_as_stated(mode, layout):
-    """cfg1: 10 frames of the 16^3 cube dropped from 0.5 (free fall, no contact)."""
-    _print_case("cfg1_as_stated", mode=mode, layout=layout)
-    scene = cfg1_plain_small()
+def test_dropped_cube_as_stated(mode, layout):
+    """10 frames of the 16^3 cube dropped from 0.5 (free fall, no contact)."""
+    _print_case("dropped_cube", mode=mode, layout=layout)
+    scene = dropped_cube()
     solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
     check_positions(scene, solver, orc)
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("mode", ["atomic", "csr"])
-def test_cfg1_contact_curve(mode, layout):
-    _print_case("cfg1_contact", mode=mode, layout=layout)
-    scene = cfg1_contact()
+def test_resting_cube_curve(mode, layout):
+    _print_case("resting_cube", mode=mode, layout=layout)
+    scene = resting_cube()
     solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
     check_positions(scene, solver, orc)
     check_residual_curve(solver, orc)
@@ -124,7 +124,7 @@ def test_split_mode(cheb, mode, layout):
     """block_neohookean_enable=False: solve_hydrostatic + solve_deviatoric with
     relaxation 1.2 and data-dependent cons_num."""
     _print_case("split_mode", cheb=cheb, mode=mode, layout=layout)
-    scene = cfg1_contact(n=8, frames=5)
+    scene = resting_cube(n=8, frames=5)
     scene.params["block_neohookean_enable"] = False
     scene.params["cheb_enable"] = cheb
     solver, orc = run_pair(scene, accumulate_mode=mode, layout=layout)
@@ -137,7 +137,7 @@ def test_split_mode(cheb, mode, layout):
 def test_monitor_errors_no_cheb(layout):
     """monitor_convergence with Chebyshev off: hydro/dev errors per iteration."""
     _print_case("monitor_no_cheb", layout=layout)
-    scene = cfg1_contact(n=8, frames=3)
+    scene = resting_cube(n=8, frames=3)
     scene.params["cheb_enable"] = False
     scene.params["monitor_convergence"] = True
     solver, orc = run_pair(scene, layout=layout)
@@ -156,7 +156,7 @@ def test_monitor_errors_no_cheb(layout):
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_csr_is_deterministic(layout):
     _print_case("csr_deterministic", layout=layout)
-    scene = cfg1_contact(n=8, frames=3)
+    scene = resting_cube(n=8, frames=3)
     a, _ = run_pair(scene, accumulate_mode="csr", layout=layout)
     b, _ = run_pair(scene, accumulate_mode="csr", layout=layout)
     assert np.array_equal(a.mesh.verts.x.to_numpy(), b.mesh.verts.x.to_numpy())
@@ -165,7 +165,7 @@ def test_csr_is_deterministic(layout):
 
 def test_graph_matches_kernels():
     _print_case("graph_vs_kernels")
-    scene = cfg1_contact(n=8, frames=2)
+    scene = resting_cube(n=8, frames=2)
     a, _ = run_pair(scene, accumulate_mode="csr", exec_path="graph")
     b, _ = run_pair(scene, accumulate_mode="csr", exec_path="kernels")
     assert np.array_equal(a.mesh.verts.x.to_numpy(), b.mesh.verts.x.to_numpy())
@@ -177,7 +177,7 @@ def test_kernel_timing_leaves_results_unchanged(layout, replays):
     """Timed projection replays restore lambda, so a timed solve is bit-identical
     to an untimed one, and the timer counts every timed launch."""
     _print_case("kernel_timing", layout=layout, replays=replays)
-    scene = cfg1_contact(n=8, frames=2)
+    scene = resting_cube(n=8, frames=2)
     a = make_solver(scene, accumulate_mode="csr", layout=layout, log_residuals=True)
     b = make_solver(scene, accumulate_mode="csr", layout=layout, log_residuals=True)
     b.set_kernel_timing(True, replays=replays)
@@ -196,7 +196,7 @@ def test_kernel_timing_leaves_results_unchanged(layout, replays):
 def test_unreordered_matches_oracle_closely():
     """Without internal reordering the flat CSR gather sums in the oracle's tet order."""
     _print_case("no_reorder")
-    scene = cfg1_contact(n=6, frames=3)
+    scene = resting_cube(n=6, frames=3)
     solver, orc = run_pair(scene, accumulate_mode="csr", reorder=False, layout="flat")
     check_positions(scene, solver, orc, tol=1e-6)
 
@@ -206,7 +206,7 @@ def test_tet_orders_match_oracle(order):
     """Every internal ordering the batch autotuner may pick meets the
     position and residual-curve gates."""
     _print_case("tet_order", order=order)
-    scene = cfg1_contact(n=8, frames=3)
+    scene = resting_cube(n=8, frames=3)
     solver, orc = run_pair(scene, accumulate_mode="csr", layout="patch", reorder=order)
     check_positions(scene, solver, orc)
     check_residual_curve(solver, orc)
@@ -237,7 +237,7 @@ def test_isolated_vertices_match_oracle(layout):
     from scenes import Scene
     from xpbd_gpu.mesh import TetMesh
     _print_case("isolated", layout=layout)
-    base = cfg1_contact(n=4, frames=3)
+    base = resting_cube(n=4, frames=3)
     extra = np.array([[3.0, 1.0, 0.0], [3.5, 1.0, 0.0], [-2.0, 0.5, 1.0]], dtype=np.float32)
     m = TetMesh(np.vstack([base.mesh.verts, extra]), base.mesh.tets)
     sc = Scene("isolated", m, base.sdfs, base.params, total_mass=base.total_mass,
@@ -253,7 +253,7 @@ def test_patch_and_flat_agree():
     """The patch layout recomputes inv_Dm/V0 from rest positions bit-exactly, so
     both layouts in deterministic mode differ only by summation order."""
     _print_case("patch_vs_flat")
-    scene = cfg1_contact(n=8, frames=3)
+    scene = resting_cube(n=8, frames=3)
     a, _ = run_pair(scene, accumulate_mode="csr", layout="patch")
     b, _ = run_pair(scene, accumulate_mode="csr", layout="flat")
     d = np.abs(a.mesh.verts.x.to_numpy() - b.mesh.verts.x.to_numpy()).max()
@@ -262,7 +262,7 @@ def test_patch_and_flat_agree():
 
 def test_sphere_pinning_and_masses():
     _print_case("sphere_pinning")
-    scene = cfg2_plain_large(n=12, frames=2, iters=20)
+    scene = jittered_box(n=12, frames=2, iters=20)
     solver = make_solver(scene)
     orc = make_oracle(scene)
     m, im = solver._get_masses()
@@ -282,7 +282,7 @@ def test_sphere_pinning_and_masses():
 def test_positions_roundtrip_host_and_device():
     _print_case("positions_roundtrip")
     import torch
-    scene = cfg1_contact(n=4, frames=1)
+    scene = resting_cube(n=4, frames=1)
     solver = make_solver(scene)
     x = solver.mesh.verts.x.to_numpy()
     rng = np.random.default_rng(1)
@@ -299,7 +299,7 @@ def test_positions_roundtrip_host_and_device():
 
 
 def test_visual_indices():
-    scene = cfg1_contact(n=2, frames=1)
+    scene = resting_cube(n=2, frames=1)
     solver = make_solver(scene)
     ind = solver.indices.to_numpy()
     tets = scene.mesh.tets
@@ -310,7 +310,7 @@ def test_visual_indices():
 
 
 def test_nan_is_reported():
-    scene = cfg1_contact(n=2, frames=1)
+    scene = resting_cube(n=2, frames=1)
     solver = make_solver(scene)
     solver.check_finite()
     x = solver.mesh.verts.x.to_numpy()
@@ -322,8 +322,8 @@ def test_nan_is_reported():
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
-def test_cfg3_stress(layout):
-    """cfg3: 32^3 grid, seeded stretch + 180-degree twist, 200 iterations.
+def test_twisted_grid(layout):
+    """32^3 grid, seeded stretch + 180-degree twist, 200 iterations.
 
     This case is ill-conditioned in f32: the f32 oracle itself ends 5e-3 x diag
     away from the f64 oracle, so a 1e-4 x diag position gate against the f32
@@ -332,8 +332,8 @@ def test_cfg3_stress(layout):
     the f32 oracle's error vs the same f64 run, and the residual curve must
     match within 1 %.
     """
-    _print_case("cfg3_stress", layout=layout)
-    scene = cfg3_stress()
+    _print_case("twisted_grid", layout=layout)
+    scene = twisted_grid()
     solver, orc = run_pair(scene, layout=layout)
     o64 = make_oracle(scene, precision=64)
     o64.frame(1)
@@ -350,10 +350,10 @@ def test_cfg3_stress(layout):
 
 @pytest.mark.slow
 @pytest.mark.parametrize("mode", ["atomic", "csr"])
-def test_cfg2_one_substep(mode):
-    """cfg2 at full size (1.57 M tets): one substep of 100 iterations."""
-    _print_case("cfg2_one_substep", mode=mode)
-    scene = cfg2_plain_large()
+def test_jittered_box_one_substep(mode):
+    """The 64^3 jittered box at full size (1.57 M tets): one substep of 100 iterations."""
+    _print_case("jittered_box_one_substep", mode=mode)
+    scene = jittered_box()
     solver, orc = run_pair(scene, substeps=1, accumulate_mode=mode)
     check_positions(scene, solver, orc)
     check_residual_curve(solver, orc)

@@ -15,7 +15,7 @@ import os
 import numpy as np
 import pytest
 
-from scenes import (batched_scenes, cfg1_contact, cfg1_plain_small, cfg2_plain_large, cfg3_stress,
+from scenes import (batched_scenes, resting_cube, dropped_cube, jittered_box, twisted_grid,
                     make_oracle)
 from test_gpu_parity import POS_TOL, check_residual_curve, make_solver
 from xpbd_gpu.batch import SceneBatch
@@ -97,30 +97,30 @@ def run_case(name, scene, frames=None, substeps=None, pos_gate=True, curve=True)
     return solver, entry
 
 
-def test_report_cfg1_as_stated():
-    solver, e = run_case("cfg1", cfg1_plain_small(), frames=10, curve=False)
-    _write("cfg1_free_fall", e)
+def test_report_dropped_cube():
+    solver, e = run_case("dropped_cube", dropped_cube(), frames=10, curve=False)
+    _write("dropped_cube_free_fall", e)
 
 
-def test_report_cfg1_contact():
-    solver, e = run_case("cfg1_contact", cfg1_contact(), frames=10)
-    _write("cfg1_contact", e)
+def test_report_resting_cube():
+    solver, e = run_case("resting_cube", resting_cube(), frames=10)
+    _write("resting_cube", e)
 
 
-def test_report_cfg2_one_frame():
-    solver, e = run_case("cfg2", cfg2_plain_large(), frames=1)
-    _write("cfg2_one_frame", e)
+def test_report_jittered_box_one_frame():
+    solver, e = run_case("jittered_box", jittered_box(), frames=1)
+    _write("jittered_box_one_frame", e)
 
 
-def test_report_cfg3_stress():
-    scene = cfg3_stress()
+def test_report_twisted_grid():
+    scene = twisted_grid()
     before = det_f(scene.init_x, scene.mesh.verts, scene.mesh.tets)
-    solver, e = run_case("cfg3", scene, frames=1, pos_gate=False)
+    solver, e = run_case("twisted_grid", scene, frames=1, pos_gate=False)
     after = det_f(solver.mesh.verts.x.to_numpy(), scene.mesh.verts, scene.mesh.tets)
     e["det_f_before"] = histogram(before)
     e["det_f_after"] = histogram(after)
     e["det_f_after_min_max"] = [float(after.min()), float(after.max())]
-    _write("cfg3_stress", e)
+    _write("twisted_grid", e)
 
 
 def _sample_errors(scenes, xs, frames):

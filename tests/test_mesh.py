@@ -43,7 +43,7 @@ def test_anisotropic_grid_and_jitter_seed():
     assert (signed_volumes(a.verts, a.tets) > 0).all()
 
 
-def test_cfg2_mesh_has_no_inverted_tets():
+def test_jittered_box_mesh_has_no_inverted_tets():
     m = box_grid(64, jitter=0.2, seed=0)
     assert m.n_verts == 274625 and m.n_tets == 1572864
     vol = signed_volumes(m.verts, m.tets)

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from scenes import cfg1_contact, cfg1_plain_small, cfg3_stress, make_oracle
+from scenes import resting_cube, dropped_cube, twisted_grid, make_oracle
 from xpbd_gpu.mesh import box_grid
 
 RNG = np.random.default_rng(1234)
@@ -45,7 +45,7 @@ def test_lame_textbook(E, nu):
     assert mu == pytest.approx(E / 2 / (1 + nu), rel=1e-12)
 
 
-def test_lame_cfg_values():
+def test_lame_values():
     lam, mu = oracle.lame(1e5, 0.45)
     assert lam == pytest.approx(310344.83, abs=0.01)
     assert mu == pytest.approx(34482.76, abs=0.01)
@@ -69,7 +69,7 @@ def test_setup_rest_state_of_grid():
     tet; inv_Dm maps the rest edges to the identity."""
     n = 4
     m = box_grid(n)
-    o = make_oracle(cfg1_plain_small(n=n), precision=64)
+    o = make_oracle(dropped_cube(n=n), precision=64)
     v0, idm, _, _ = o.tet_state()
     np.testing.assert_allclose(v0, (1.0 / n) ** 3 / 6.0, rtol=1e-12)
     X = m.verts.astype(np.float64) + np.array([0.0, 0.5, 0.0])
@@ -85,7 +85,7 @@ def test_lumped_mass_closed_form():
     """Sum m = total_mass; on a regular grid an interior vertex touches 24 tets
     (mass = 24 V/4 scaled), a corner on the Kuhn diagonal touches 6."""
     n = 4
-    s = cfg1_plain_small(n=n, drop=0.5)
+    s = dropped_cube(n=n, drop=0.5)
     o = make_oracle(s, precision=64)
     m, im = o.mass()
     assert m.sum() == pytest.approx(1000.0, rel=1e-12)
@@ -99,8 +99,8 @@ def test_lumped_mass_closed_form():
 
 
 def test_fixed_sdf_pins_initially_inside():
-    from scenes import cfg2_plain_large
-    s = cfg2_plain_large(n=8, frames=1, iters=2)
+    from scenes import jittered_box
+    s = jittered_box(n=8, frames=1, iters=2)
     o = make_oracle(s, precision=64)
     _, im = o.mass()
     X = s.mesh.verts.astype(np.float64)
@@ -236,7 +236,7 @@ def test_omega_recurrence_fixed_point():
 
 
 def test_oracle_omega_schedule_in_log():
-    s = cfg1_contact(n=4, frames=1)
+    s = resting_cube(n=4, frames=1)
     o = make_oracle(s, precision=64)
     o.frame(1)
     L = o.log()
@@ -262,7 +262,7 @@ def test_oracle_omega_schedule_in_log():
 
 def test_free_fall_is_ballistic_with_damping():
     """No strain: v_{k+1} = (v_k + g dt) e^{-5 dt}, x_{k+1} = x_k + (v_k + g dt) dt."""
-    s = cfg1_plain_small(n=2, drop=5.0, frames=3)
+    s = dropped_cube(n=2, drop=5.0, frames=3)
     o = make_oracle(s, precision=64)
     o.frame(3)
     dt = s.params["dt"] / s.params["substeps_num"]
@@ -352,7 +352,7 @@ def test_two_tet_jacobi_average_brute_force():
 # ---- properties of full runs ------------------------------------------------------
 
 def test_threads_do_not_change_bits():
-    s = cfg1_contact(n=6, frames=2)
+    s = resting_cube(n=6, frames=2)
     a = make_oracle(s, threads=1)
     b = make_oracle(s, threads=4)
     a.frame(2)
@@ -362,7 +362,7 @@ def test_threads_do_not_change_bits():
 
 
 def test_f32_tracks_f64():
-    s = cfg1_contact(n=6, frames=3)
+    s = resting_cube(n=6, frames=3)
     a = make_oracle(s, precision=32)
     b = make_oracle(s, precision=64)
     a.frame(3)
@@ -371,7 +371,7 @@ def test_f32_tracks_f64():
 
 
 def test_residual_zero_at_rest():
-    s = cfg1_plain_small(n=3, drop=5.0, frames=1)
+    s = dropped_cube(n=3, drop=5.0, frames=1)
     s.gravity = (0.0, 0.0, 0.0)
     o = make_oracle(s, precision=64)
     o.set_gravity([0, 0, 0])
@@ -381,7 +381,7 @@ def test_residual_zero_at_rest():
 
 
 def test_stress_run_stays_finite_and_relaxes():
-    s = cfg3_stress(n=6, iters=200)
+    s = twisted_grid(n=6, iters=200)
     o = make_oracle(s, precision=64)
     o.frame(1)
     L = o.log()
