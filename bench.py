@@ -420,6 +420,7 @@ def main():
             "exec_path": args.exec_path,
         },
         "hbm_gbps_algorithmic": gbps,
+        "hbm_frac_frame": round(gbps / HBM_PEAK_GBPS, 4),  # all kernels of a frame together
         "b_alg_per_iteration": b_alg_iteration(T, V),
         "step_latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                             "p99": float(np.percentile(lat, 99) * 1e3)},
