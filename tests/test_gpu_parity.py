@@ -359,6 +359,32 @@ def test_jittered_box_one_substep(mode):
     check_residual_curve(solver, orc)
 
 
+@pytest.mark.slow
+def test_jittered_box_ten_frames_properties():
+    """The 64^3 jittered box as stated (10 frames of 10 x 100 iterations) at
+    full size, checked through properties that need no oracle run of that
+    length: two solvers give the same bits; every position stays finite; every
+    substep lowers r_hat from its first to its last iteration; rho_hat stays in
+    its clamp.  (Collision runs before the iterations, as in the reference, so
+    vertices may end a substep slightly inside the plane: not a property.)"""
+    _print_case("jittered_box_ten_frames")
+    scene = jittered_box()
+    a = make_solver(scene, log_residuals=True)
+    b = make_solver(scene)
+    for _ in range(scene.frames):
+        a.solve()
+        b.solve()
+    xa, xb = a.mesh.verts.x.to_numpy(), b.mesh.verts.x.to_numpy()
+    assert np.array_equal(xa, xb)
+    assert np.all(np.isfinite(xa))
+    rows = residual_rows(a)
+    iters = scene.params["constraints_iter"]
+    assert rows.shape[0] == scene.frames * scene.params["substeps_num"] * iters
+    r = rows[:, 3].reshape(-1, iters)
+    assert np.all(r[:, -1] < r[:, 0])
+    assert np.all((rows[:, 4] >= 0.00001 - 1e-12) & (rows[:, 4] <= 0.99999 + 1e-12))
+
+
 def test_single_tet_free_fall_matches_closed_form():
     """An unstressed tet in free fall: x_{k+1} = x_k + (v_k + g dt) dt and
     v_{k+1} = (v_k + g dt) e^{-5 dt} (xpbd_base.py:139-160)."""
