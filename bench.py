@@ -134,6 +134,31 @@ def summarize_stats(folder: str) -> str:
     return "\n".join(lines) + "\n"
 
 
+BATCHED_STATS_COMMAND = ("cd <repo root>; export TMPDIR=/tmp; rocprofv3 --kernel-trace --stats "
+                         "--output-format csv -d <D> -o run -- python3 bench.py --batched-only "
+                         "--no-tune; python3 bench.py --summarize-batched-stats <D>")
+
+
+def summarize_batched_stats(folder: str) -> str:
+    """Per-kernel launch counts and durations of the batched measurement (256
+    ragged scenes, default launch options) from rocprofv3 --kernel-trace --stats."""
+    import csv
+    with open(os.path.join(folder, "run_kernel_stats.csv")) as fh:
+        stats = list(csv.DictReader(fh))
+    lines = ["# rocprofv3 kernel statistics of the batched measurement: 256 box scenes, per-axis",
+             "# resolution uniform in 6..40 (seed 0), 19 999 224 tets, 5 substeps x 50 iterations",
+             "# per frame, default launch options (one graph per substep, bisection order),",
+             "# 1 warmup + 3 timed batched frames.",
+             "# Made by: " + BATCHED_STATS_COMMAND, "",
+             f"{'kernel':30s} {'calls':>7s} {'avg us':>9s} {'min us':>9s} {'max us':>9s} {'%':>7s}"]
+    for r in stats:
+        name = r["Name"].split("(")[0].replace("void ", "").strip()
+        lines.append(f"{name:30s} {int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:9.2f} "
+                     f"{float(r['MinNs']) / 1e3:9.2f} {float(r['MaxNs']) / 1e3:9.2f} "
+                     f"{float(r['Percentage']):7.2f}")
+    return "\n".join(lines) + "\n"
+
+
 def committed_traffic(path: str = PMC_SUMMARY):
     """The projection kernel's measured bytes per launch from the committed
     counter summary, or None when it is absent."""
@@ -166,7 +191,7 @@ def cpu_baseline(scene, substeps: int, threads: int) -> dict:
                       f"OpenMP over tets"}
 
 
-def batched_measurement(n_scenes: int, frames: int, tune: bool) -> dict:
+def batched_measurement(n_scenes: int, frames: int, tune: bool, sequential: bool = True) -> dict:
     """256 ragged scenes in one batched launch sequence vs the same scenes
     solved one after another with plain per-scene solves."""
     from xpbd_gpu.batch import SceneBatch, autotune, autotune_ordering, default_candidates
@@ -204,6 +229,13 @@ def batched_measurement(n_scenes: int, frames: int, tune: bool) -> dict:
             lat.append(time.perf_counter() - s0)
         el = time.perf_counter() - t0
     batched = T * iters * subs * frames / el
+    workload = (f"{len(solvers)} box scenes, per-axis resolution uniform in 6..40 (seed 0), "
+                f"{T} tets in total, {subs} substeps x {iters} iterations per frame")
+    if not sequential:
+        return {"workload": workload, "tet_iterations_per_s": batched,
+                "frame_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
+                             "p99": float(np.percentile(lat, 99) * 1e3)},
+                "tuned_opts": dict(opts, tet_order=order)}
     # sequential plain solves of the same scenes, one frame each
     for s in solvers[:4]:
         s.solve()  # warm-up of per-scene graphs
@@ -216,8 +248,7 @@ def batched_measurement(n_scenes: int, frames: int, tune: bool) -> dict:
     seq_el = time.perf_counter() - t0
     sequential = T * iters * subs / seq_el
     return {
-        "workload": f"{len(solvers)} box scenes, per-axis resolution uniform in 6..40 (seed 0), "
-                    f"{T} tets in total, {subs} substeps x {iters} iterations per frame",
+        "workload": workload,
         "tet_iterations_per_s": batched,
         "sequential_plain_tet_iterations_per_s": sequential,
         "speedup_vs_sequential": batched / sequential,
@@ -321,8 +352,15 @@ def main():
                     help="only summarize a rocprofv3 --kernel-trace --stats folder")
     ap.add_argument("--timing-replays", type=int, default=8,
                     help="back-to-back projection replays per event pair for the roofline")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the batched frames (no sequential comparison), print JSON")
+    ap.add_argument("--summarize-batched-stats", default="",
+                    help="only summarize a rocprofv3 --stats folder of a --batched-only run")
     args = ap.parse_args()
 
+    if args.summarize_batched_stats:
+        sys.stdout.write(summarize_batched_stats(args.summarize_batched_stats))
+        return
     if args.summarize_pmc:
         sys.stdout.write(summarize_pmc(args.summarize_pmc))
         return
@@ -347,6 +385,11 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
+
+    if args.batched_only:
+        print(json.dumps(batched_measurement(args.batched_scenes, args.batched_frames,
+                                             not args.no_tune, sequential=False)), flush=True)
+        return
 
     from xpbd_gpu.scenes import jittered_box, make_solver
 
