@@ -26,6 +26,16 @@
 //
 // Built with -ffp-contract=off so that no fused multiply-add changes the op
 // order written below.
+//
+// Pins.  Every function is checked against closed forms, brute force or
+// library routines by tests/test_oracle_pins.py (Lame constants, rest shape,
+// lumped mass and pinning, free fall, each collider, gradients against finite
+// differences, the 2x2 solve and its fallback, the Jacobi average, the
+// residual at rest, the omega schedule, f32 against f64) and its outputs are
+// frozen in tests/golden.  Parity unpinned: the operation order inside Taichi's
+// Matrix.inverse()/determinant() (restated here by their cofactor formulas)
+// and the order of the reference's float-atomic scatter (fixed here to
+// ascending tet index); both only change round-off.
 
 #include <algorithm>
 #include <cmath>
