@@ -154,6 +154,45 @@ def test_monitor_errors_no_cheb(layout):
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("iters", [0, 1, 2, 5, 6])
+def test_iteration_counts_at_the_sequence_edges(iters, layout):
+    """constraints_iter at the edges of the iteration sequence: none (the
+    substep is predict, collide, advance), one (the last iteration is also the
+    first, so the blend after it uses omega = 1 from the standalone reduce),
+    two, and the Chebyshev warmup boundary (omega = 2 / (2 - rho^2) at iteration
+    5, neohookean_solver.py:537-546)."""
+    _print_case("iteration_counts", iters=iters, layout=layout)
+    scene = resting_cube(n=4, frames=3)
+    scene.params["constraints_iter"] = iters
+    solver, orc = run_pair(scene, layout=layout)
+    check_positions(scene, solver, orc, tol=1e-5)
+    rows = residual_rows(solver)
+    assert rows.shape[0] == orc.log().shape[0] == 3 * scene.params["substeps_num"] * iters
+    if iters:
+        check_residual_curve(solver, orc)
+        # the device's rho_hat and omega columns follow the reference's
+        # recurrence (neohookean_solver.py:461-466, 492-504, 537-546) applied
+        # to the device's own r_hat column.  (Against the oracle they agree
+        # only loosely: rho_hat averages r_hat ratios taken down to round-off.)
+        rho, warmup = 0.999, scene.params["cheb_warmup"]
+        for s0 in range(0, rows.shape[0], iters):
+            prev, omega = None, 1.0
+            for it in range(iters):
+                r = rows[s0 + it, 3]
+                if prev is not None and prev > 1e-12:
+                    rho = 0.8 * rho + 0.2 * min(max(r / (prev + 1e-12), 1e-5), 0.99999)
+                prev = r
+                if it < warmup:
+                    omega = 1.0
+                elif it == warmup:
+                    omega = 2.0 / (2.0 - rho * rho)
+                else:
+                    omega = 4.0 / (4.0 - rho * rho * omega)
+                assert rows[s0 + it, 4] == pytest.approx(rho, rel=1e-12)
+                assert rows[s0 + it, 5] == pytest.approx(omega, rel=1e-12)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
 def test_csr_is_deterministic(layout):
     _print_case("csr_deterministic", layout=layout)
     scene = resting_cube(n=8, frames=3)
