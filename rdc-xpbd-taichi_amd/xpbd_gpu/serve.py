@@ -107,6 +107,7 @@ class Server:
         self.solver_opts = dict(solver_opts or {})
         self.resident = collections.OrderedDict()  # scene -> solver, least recently used first
         self.bytes = {}
+        self.footprint = {}  # scene -> device bytes, measured when first built
         self.parked = {}  # evicted scenes' state, restored when they return
         self.rebuilds = 0
         self.substeps_run = 0
@@ -120,21 +121,32 @@ class Server:
         return sum(self.bytes.values())
 
     def make_resident(self, k: int, busy=()) -> bool:
-        """Creates scene k if absent, evicting idle scenes while over budget."""
+        """Makes scene k resident if it fits the budget once idle scenes are
+        evicted (least recently used first); otherwise changes nothing and
+        returns False.  A scene's device footprint is measured when it is
+        first built and remembered, so a scene that has to wait is not built
+        again at every check."""
         if k in self.resident:
             self.resident.move_to_end(k)
             return True
-        s = make_solver(self.specs[k], **self.solver_opts)
-        need = scene_bytes(s)
+        s = None
+        if k not in self.footprint:
+            s = make_solver(self.specs[k], **self.solver_opts)
+            self.footprint[k] = scene_bytes(s)
+        need = self.footprint[k]
+        idle = sum(b for v, b in self.bytes.items() if v not in busy)
+        if self.used_bytes - idle + need > self.budget:
+            if s is not None:
+                s.close()
+            return False
         while self.used_bytes + need > self.budget:
-            victim = next((v for v in self.resident if v not in busy), None)
-            if victim is None:
-                del s
-                return False
+            victim = next(v for v in self.resident if v not in busy)
             self.parked[victim] = self.resident[victim].save_state()
-            del self.resident[victim]
+            self.resident.pop(victim).close()
             del self.bytes[victim]
             self.evictions += 1
+        if s is None:
+            s = make_solver(self.specs[k], **self.solver_opts)
         if k in self.parked:
             s.load_state(self.parked.pop(k))
         self.resident[k] = s
