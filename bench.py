@@ -327,6 +327,42 @@ def served_measurement(n_scenes: int, rate: float, seconds: float) -> dict:
     }
 
 
+def small_configs_measurement(frames: int = 10) -> dict:
+    """Throughput of the two smaller single-scene configurations: the 16^3
+    cube (24 576 tets, 5 substeps x 30 iterations per frame; as stated it
+    falls from 0.5, and the same cube resting on the plane is the contact
+    case) and the 32^3 twisted grid (196 608 tets, one substep of 200
+    iterations per frame, relaxing from its stretched and twisted start).
+    Each runs one untimed frame, then `frames` timed frames."""
+    from xpbd_gpu.scenes import dropped_cube, make_solver, resting_cube, twisted_grid
+
+    out = {}
+    for name, scene in (("dropped_cube16", dropped_cube()), ("resting_cube16", resting_cube()),
+                        ("twisted_grid32", twisted_grid())):
+        solver = make_solver(scene)
+        T, V = scene.mesh.n_tets, scene.mesh.n_verts
+        it_per_frame = scene.params["substeps_num"] * scene.params["constraints_iter"]
+        solver.solve()
+        solver.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            s0 = time.perf_counter()
+            solver.solve()
+            lat.append(time.perf_counter() - s0)
+        el = time.perf_counter() - t0
+        solver.check_finite()
+        out[name] = {"tets": T, "verts": V, "iterations_per_frame": it_per_frame,
+                     "frames": frames, "ms_per_frame": el / frames * 1e3,
+                     "frame_p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "tet_iterations_per_s": T * it_per_frame * frames / el,
+                     "us_per_iteration": el / (frames * it_per_frame) * 1e6,
+                     "hbm_gbps_algorithmic":
+                         b_alg_iteration(T, V) * it_per_frame * frames / el / 1e9}
+        solver.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -476,6 +512,7 @@ def main():
     }
     del solver
     if not args.plain_only:
+        out["small_configs"] = small_configs_measurement()
         out["batched"] = batched_measurement(args.batched_scenes, args.batched_frames,
                                              not args.no_tune)
         out["served"] = served_measurement(args.serve_scenes, args.serve_rate,
