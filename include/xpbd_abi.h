@@ -202,12 +202,14 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets);
 int xpbd_set_exec_path(xpbd_solver* s, int32_t path);
 
 /* Kernel timing (no reference counterpart).  mode 0: off.  mode 1: every
- * launch of the tet projection kernel is bracketed by HIP events on the
- * solver's stream.  mode r in [2, 1000]: before each projection, r replays of
- * it run back to back between one event pair (lambda is saved before and
- * restored after, so results are unchanged); this removes the per-pair event
- * overhead from the average.  xpbd_get_kernel_time returns the summed
- * milliseconds and launch count since the last reset (synchronizes). */
+ * launch of the tet projection kernel is timed by HIP events on the solver's
+ * stream; in the patch layout the launch records them itself
+ * (hipExtLaunchKernel: the kernel's own start and end), in the flat layout
+ * they bracket it.  mode r in [2, 1000]: before each projection, r replays of
+ * it run back to back, each timed the same way (flat layout: one event pair
+ * around all r); lambda is saved before and restored after, so results are
+ * unchanged.  xpbd_get_kernel_time returns the summed milliseconds and launch
+ * count since the last reset (synchronizes). */
 int xpbd_set_kernel_timing(xpbd_solver* s, int32_t enabled);
 int xpbd_get_kernel_time(xpbd_solver* s, double* total_ms, int64_t* launches);
 

@@ -455,9 +455,10 @@ struct xpbd_solver {
   hipGraphExec_t gexec = nullptr;
   int graph_key = -1;
 
-  // kernel timing: 0 off; 1 brackets each projection launch with an event
-  // pair; r > 1 times r back-to-back replays of each projection (lambda saved
-  // before and restored after, so the solve itself is unchanged)
+  // kernel timing: 0 off; 1 times each projection launch with an event pair
+  // (recorded by the dispatch itself in the patch layout); r > 1 times r
+  // back-to-back replays of each projection (lambda saved before and restored
+  // after, so the solve itself is unchanged)
   int timing = 0;
   DevBuf<float2> lam_save;
   std::vector<TimedSpan> ev_pending, ev_free;
@@ -770,7 +771,7 @@ int upload_inv_m(xpbd_solver* s) {
   return XPBD_OK;
 }
 
-int events_begin(xpbd_solver* s, TimedSpan& ev) {
+int events_acquire(xpbd_solver* s, TimedSpan& ev) {
   if (!s->ev_free.empty()) {
     ev = s->ev_free.back();
     s->ev_free.pop_back();
@@ -778,6 +779,11 @@ int events_begin(xpbd_solver* s, TimedSpan& ev) {
     HIP_TRY(hipEventCreate(&ev.begin));
     HIP_TRY(hipEventCreate(&ev.end));
   }
+  return XPBD_OK;
+}
+
+int events_begin(xpbd_solver* s, TimedSpan& ev) {
+  TRY(events_acquire(s, ev));
   HIP_TRY(hipEventRecord(ev.begin, s->stream));
   return XPBD_OK;
 }
@@ -821,6 +827,17 @@ int enqueue_iterations(xpbd_solver* s, bool capturing) {
       HIP_TRY(xpbd::launch_project(T, s->tmap.p, s->ntb, s->bs_t, it == 0, s->stream));
     return XPBD_OK;
   };
+  // one patch projection timed by events that its own dispatch records (the
+  // kernel's start and end, without the gaps between launches)
+  auto timed_project = [&](int it) -> int {
+    TimedSpan ev;
+    TRY(events_acquire(s, ev));
+    HIP_TRY(xpbd::launch_patch_project(T, s->pitems.p, s->prange_proj.p, s->grid_proj, it,
+                                       false, s->stream, ev.begin, ev.end));
+    ev.launches = 1;
+    s->ev_pending.push_back(ev);
+    return XPBD_OK;
+  };
   const size_t lam_bytes = sizeof(float2) * s->lam.n;
   for (int it = 0; it < S.iters; ++it) {
     TimedSpan ev;
@@ -831,15 +848,23 @@ int enqueue_iterations(xpbd_solver* s, bool capturing) {
       // replays differ from the real launch only in the lambda they leave.
       HIP_TRY(hipMemcpyAsync(s->lam_save.p, S.lam, lam_bytes, hipMemcpyDeviceToDevice,
                              s->stream));
-      TRY(events_begin(s, ev));
-      for (int r = 0; r < reps; ++r) TRY(project(it));
-      TRY(events_end(s, ev, reps));
+      if (patch) {
+        for (int r = 0; r < reps; ++r) TRY(timed_project(it));
+      } else {
+        TRY(events_begin(s, ev));
+        for (int r = 0; r < reps; ++r) TRY(project(it));
+        TRY(events_end(s, ev, reps));
+      }
       HIP_TRY(hipMemcpyAsync(S.lam, s->lam_save.p, lam_bytes, hipMemcpyDeviceToDevice,
                              s->stream));
     }
-    if (reps == 1) TRY(events_begin(s, ev));
-    TRY(project(it));
-    if (reps == 1) TRY(events_end(s, ev, 1));
+    if (reps == 1 && patch) {
+      TRY(timed_project(it));
+    } else {
+      if (reps == 1) TRY(events_begin(s, ev));
+      TRY(project(it));
+      if (reps == 1) TRY(events_end(s, ev, 1));
+    }
     if (patch)
       HIP_TRY(xpbd::launch_patch_average(T, s->vmap.p, s->nvb, s->bs_v, it, s->stream));
     else

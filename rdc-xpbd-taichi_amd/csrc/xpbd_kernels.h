@@ -20,7 +20,8 @@ hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nbl
 // multi_scene: the patch ranges may cross scenes (batches)
 hipError_t launch_patch_project(const SceneDev* scenes, const PatchItem* items,
                                 const int2* ranges, int nblocks, int it, bool multi_scene,
-                                hipStream_t st);
+                                hipStream_t st, hipEvent_t start = nullptr,
+                                hipEvent_t stop = nullptr);
 hipError_t launch_patch_average(const SceneDev* scenes, const WorkItem* vmap, int nblocks,
                                 int bs, int it, hipStream_t st);
 hipError_t launch_patch_residual(const SceneDev* scenes, const PatchItem* items,

@@ -17,6 +17,8 @@
 #include "xpbd_device.h"
 #include "xpbd_kernels.h"
 
+#include <hip/hip_ext.h>
+
 namespace xpbd {
 
 namespace {
@@ -963,8 +965,16 @@ hipError_t launch_residual(const SceneDev* scenes, const WorkItem* tmap, int nbl
 }
 hipError_t launch_patch_project(const SceneDev* scenes, const PatchItem* items,
                                 const int2* ranges, int nblocks, int it, bool multi_scene,
-                                hipStream_t st) {
+                                hipStream_t st, hipEvent_t start, hipEvent_t stop) {
   if (nblocks <= 0) return hipSuccess;
+  if (start || stop) {
+    // events recorded by the dispatch itself: the kernel's own start and end
+    void* args[] = {&scenes, &items, &ranges, &it};
+    const void* fn = multi_scene ? reinterpret_cast<const void*>(&k_patch_project<true>)
+                                 : reinterpret_cast<const void*>(&k_patch_project<false>);
+    return hipExtLaunchKernel(fn, dim3(nblocks), dim3(kPatchThreads), args, 0, st, start, stop,
+                              0);
+  }
   if (multi_scene)
     k_patch_project<true><<<nblocks, kPatchThreads, 0, st>>>(scenes, items, ranges, it);
   else
