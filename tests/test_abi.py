@@ -79,3 +79,28 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_abi, "_lib", None)
     with pytest.raises(ImportError):
         _abi.load()
+
+
+@pytest.mark.skipif(not _abi.library_available(), reason="libxpbd_hip.so not built")
+@pytest.mark.parametrize("bad", [4, -1])
+def test_out_of_range_vertex_index_is_rejected_before_gpu_work(bad):
+    """A tet that names a vertex outside [0, n_verts) is refused on the host,
+    before any device call, instead of reaching the kernels' gathers."""
+    import numpy as np
+    lib = _abi.load()
+    d = _abi.XpbdDesc()
+    lib.xpbd_desc_init(ctypes.byref(d))
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=np.float32)
+    t = np.array([[0, 1, 2, 3], [0, 1, 2, bad]], dtype=np.int32)
+    sdf = _abi.XpbdSdf()
+    sdf.kind = _abi.SDF_PLANE
+    sdf.dir[1] = 1.0
+    d.n_verts, d.n_tets = 4, 2
+    d.rest_positions = v.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    d.tets = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    d.n_sdf = 1
+    d.sdfs = ctypes.pointer(sdf)
+    h = ctypes.c_void_p()
+    assert lib.xpbd_create(ctypes.byref(d), ctypes.byref(h)) == _abi.XPBD_EINVAL
+    assert "tet 1 references vertex" in _abi.last_error()
+    assert not h.value
