@@ -143,6 +143,37 @@ BATCHED_STATS_COMMAND = ("cd <repo root>; export TMPDIR=/tmp; rocprofv3 --kernel
                          "--no-tune; python3 bench.py --summarize-batched-stats <D>")
 
 
+SMALL_STATS_COMMAND = ("cd <repo root>; export TMPDIR=/tmp; rocprofv3 --kernel-trace --stats "
+                       "--output-format csv -d <D> -o run -- python3 bench.py --small-only; "
+                       "python3 bench.py --summarize-small-stats <D>")
+
+
+def _kernel_stats_table(folder: str, header: list) -> str:
+    import csv
+    with open(os.path.join(folder, "run_kernel_stats.csv")) as fh:
+        stats = list(csv.DictReader(fh))
+    lines = header + ["",
+                      f"{'kernel':30s} {'calls':>7s} {'avg us':>9s} {'min us':>9s} {'max us':>9s}"
+                      f" {'%':>7s}"]
+    for r in stats:
+        name = r["Name"].split("(")[0].replace("void ", "").strip()
+        lines.append(f"{name:30s} {int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:9.2f} "
+                     f"{float(r['MinNs']) / 1e3:9.2f} {float(r['MaxNs']) / 1e3:9.2f} "
+                     f"{float(r['Percentage']):7.2f}")
+    return "\n".join(lines) + "\n"
+
+
+def summarize_small_stats(folder: str) -> str:
+    """Per-kernel launch counts and durations of the small-configuration
+    measurement (the 16^3 cube falling and resting, the 32^3 twisted grid)."""
+    return _kernel_stats_table(folder, [
+        "# rocprofv3 kernel statistics of the small single-scene configurations: the 16^3 cube",
+        "# (24 576 tets, 5 substeps x 30 iterations per frame) falling and resting on the plane,",
+        "# and the 32^3 twisted grid (196 608 tets, 200 iterations per frame); each 1 untimed +",
+        "# 10 timed frames.  Kernels of both sizes are pooled per name.",
+        "# Made by: " + SMALL_STATS_COMMAND])
+
+
 def summarize_batched_stats(folder: str) -> str:
     """Per-kernel launch counts and durations of the batched measurement (256
     ragged scenes, default launch options) from rocprofv3 --kernel-trace --stats."""
@@ -392,6 +423,10 @@ def main():
                     help="only summarize a rocprofv3 --kernel-trace --stats folder")
     ap.add_argument("--timing-replays", type=int, default=8,
                     help="back-to-back projection replays per event pair for the roofline")
+    ap.add_argument("--small-only", action="store_true",
+                    help="only measure the small single-scene configurations, print JSON")
+    ap.add_argument("--summarize-small-stats", default="",
+                    help="only summarize a --kernel-trace --stats folder of --small-only")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched frames (no sequential comparison), print JSON")
     ap.add_argument("--summarize-batched-stats", default="",
@@ -400,6 +435,9 @@ def main():
 
     if args.summarize_batched_stats:
         sys.stdout.write(summarize_batched_stats(args.summarize_batched_stats))
+        return
+    if args.summarize_small_stats:
+        sys.stdout.write(summarize_small_stats(args.summarize_small_stats))
         return
     if args.summarize_pmc:
         sys.stdout.write(summarize_pmc(args.summarize_pmc))
@@ -425,6 +463,10 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
+
+    if args.small_only:
+        print(json.dumps(small_configs_measurement()), flush=True)
+        return
 
     if args.batched_only:
         print(json.dumps(batched_measurement(args.batched_scenes, args.batched_frames,
