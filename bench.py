@@ -177,21 +177,12 @@ def summarize_small_stats(folder: str) -> str:
 def summarize_batched_stats(folder: str) -> str:
     """Per-kernel launch counts and durations of the batched measurement (256
     ragged scenes, default launch options) from rocprofv3 --kernel-trace --stats."""
-    import csv
-    with open(os.path.join(folder, "run_kernel_stats.csv")) as fh:
-        stats = list(csv.DictReader(fh))
-    lines = ["# rocprofv3 kernel statistics of the batched measurement: 256 box scenes, per-axis",
-             "# resolution uniform in 6..40 (seed 0), 19 999 224 tets, 5 substeps x 50 iterations",
-             "# per frame, default launch options (one graph per substep, bisection order),",
-             "# 1 warmup + 3 timed batched frames.",
-             "# Made by: " + BATCHED_STATS_COMMAND, "",
-             f"{'kernel':30s} {'calls':>7s} {'avg us':>9s} {'min us':>9s} {'max us':>9s} {'%':>7s}"]
-    for r in stats:
-        name = r["Name"].split("(")[0].replace("void ", "").strip()
-        lines.append(f"{name:30s} {int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:9.2f} "
-                     f"{float(r['MinNs']) / 1e3:9.2f} {float(r['MaxNs']) / 1e3:9.2f} "
-                     f"{float(r['Percentage']):7.2f}")
-    return "\n".join(lines) + "\n"
+    return _kernel_stats_table(folder, [
+        "# rocprofv3 kernel statistics of the batched measurement: 256 box scenes, per-axis",
+        "# resolution uniform in 6..40 (seed 0), 19 999 224 tets, 5 substeps x 50 iterations",
+        "# per frame, default launch options (one graph per substep, bisection order),",
+        "# 1 warmup + 3 timed batched frames.",
+        "# Made by: " + BATCHED_STATS_COMMAND])
 
 
 def committed_traffic(path: str = PMC_SUMMARY):
