@@ -465,3 +465,24 @@ def test_cube_and_hang_colliders(layout):
     x = solver.mesh.verts.x.to_numpy()
     held = np.where(np.linalg.norm(m.verts - np.array([0.0, 1.3, 0.0]), axis=1) < 1e-6)[0]
     assert np.allclose(x[held], m.verts[held])  # pinned by the hang collider
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_non_fixed_plane_and_sphere(layout):
+    """A 5^3 box dropped onto a non-fixed plane with a non-fixed sphere rising
+    through it: nothing is pinned, and a vertex inside a collider is pushed out
+    only while it moves inward (SdfModel.check, sdf.py:13-27)."""
+    from xpbd_gpu.scenes import Scene, base_params, plane, sphere
+    _print_case("non_fixed_plane_sphere", layout=layout)
+    m = box_grid(5, origin=(0.0, 0.15, 0.0))
+    sdfs = [plane(0.0, fixed=False), sphere((0.5, 0.0, 0.5), 0.3, fixed=False)]
+    scene = Scene("non_fixed", m, sdfs,
+                  base_params(young_modulus=1e5, poisson_ratio=0.45, substeps_num=5,
+                              constraints_iter=30),
+                  total_mass=1000.0, frames=6)
+    solver, orc = run_pair(scene, layout=layout)
+    assert np.all(solver.mesh.verts.inv_m.to_numpy() > 0)  # no vertex pinned
+    check_positions(scene, solver, orc)
+    check_residual_curve(solver, orc)
+    x = solver.mesh.verts.x.to_numpy()
+    assert x[:, 1].min() > -0.05  # held up by the plane and the sphere
