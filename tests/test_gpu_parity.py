@@ -486,3 +486,38 @@ def test_non_fixed_plane_and_sphere(layout):
     check_residual_curve(solver, orc)
     x = solver.mesh.verts.x.to_numpy()
     assert x[:, 1].min() > -0.05  # held up by the plane and the sphere
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_parameters_changed_between_frames(layout):
+    """Material and iteration count changed on a live solver through its
+    attributes (as the reference's callers may set them) take effect from the
+    next frame: the device run matches an oracle that continues from the same
+    state with the new constructor arguments.  Raising constraints_iter also
+    grows the per-iteration log and recaptures the substep graph."""
+    _print_case("params_between_frames", layout=layout)
+    scene = resting_cube(n=4, frames=2)
+    solver = make_solver(scene, log_residuals=True, layout=layout)
+    first = make_oracle(scene)
+    for _ in range(2):
+        solver.solve()
+        first.frame(1)
+    solver.constraints_iter = 40
+    solver.young_modulus = 3e4
+    solver.cheb_warmup = 8
+    second = make_oracle(scene, constraints_iter=40, young_modulus=3e4, cheb_warmup=8)
+    second.x = first.x
+    second.v = first.v
+    second.rho_hat = first.rho_hat
+    for _ in range(2):
+        solver.solve()
+        second.frame(1)
+    check_positions(scene, solver, second, tol=1e-5)
+    rows = residual_rows(solver)
+    subs = scene.params["substeps_num"]
+    assert rows.shape[0] == 2 * subs * 30 + 2 * subs * 40
+    tail = rows[2 * subs * 30:]
+    o = second.log()
+    assert tail.shape[0] == o.shape[0]
+    assert np.array_equal(tail[:, 2], o[:, 2])  # iteration indices 0..39 per substep
+    assert np.all(tail[tail[:, 2] < 8, 5] == 1.0)  # the new warmup
