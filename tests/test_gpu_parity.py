@@ -521,3 +521,23 @@ def test_parameters_changed_between_frames(layout):
     assert tail.shape[0] == o.shape[0]
     assert np.array_equal(tail[:, 2], o[:, 2])  # iteration indices 0..39 per substep
     assert np.all(tail[tail[:, 2] < 8, 5] == 1.0)  # the new warmup
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_scale_and_offset(layout):
+    """The constructor's scale and offset (initialize, neohookean_solver.py:97-107:
+    x = X * scale + offset before the rest shape is taken): the device solver
+    and the oracle place and size the mesh alike, and the run matches."""
+    _print_case("scale_offset", layout=layout)
+    scene = resting_cube(n=4, frames=3)
+    scene.params["scale"] = 0.5
+    scene.params["offset"] = (0.2, 0.0, -0.1)
+    solver, orc = run_pair(scene, frames=0)
+    x0 = solver.mesh.verts.x.to_numpy()
+    expect = scene.mesh.verts.astype(np.float32) * np.float32(0.5) + \
+        np.array([0.2, 0.0, -0.1], dtype=np.float32)
+    assert np.allclose(x0, expect, atol=1e-7)
+    assert np.allclose(x0, orc.x, atol=1e-7)
+    solver, orc = run_pair(scene, layout=layout)
+    check_positions(scene, solver, orc, tol=1e-5)
+    check_residual_curve(solver, orc)
