@@ -541,3 +541,38 @@ def test_scale_and_offset(layout):
     solver, orc = run_pair(scene, layout=layout)
     check_positions(scene, solver, orc, tol=1e-5)
     check_residual_curve(solver, orc)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_collider_moved_between_frames(layout):
+    """A non-fixed sphere collider moved between frames through its field
+    (`sdf.center[None] = ...`, as a kinematic obstacle would be): the device
+    run matches an oracle that continues from the same state with the sphere
+    at its new centre.  (Non-fixed colliders, so no pinning is involved:
+    moving a collider does not re-pin vertices.)"""
+    from xpbd_gpu.scenes import Scene, base_params, plane, sphere
+    _print_case("collider_moved", layout=layout)
+    m = box_grid(4, origin=(0.0, 0.1, 0.0))
+    before = [plane(0.0, fixed=False), sphere((0.5, -0.15, 0.5), 0.2, fixed=False)]
+    after = [plane(0.0, fixed=False), sphere((0.5, 0.0, 0.5), 0.2, fixed=False)]
+    prm = base_params(young_modulus=1e5, poisson_ratio=0.45, substeps_num=5,
+                      constraints_iter=30)
+    scene = Scene("moved", m, before, prm, total_mass=1000.0, frames=2)
+    solver = make_solver(scene, layout=layout)
+    first = make_oracle(scene)
+    for _ in range(2):
+        solver.solve()
+        first.frame(1)
+    solver.sdfs[1].center[None] = [0.5, 0.0, 0.5]
+    second = make_oracle(Scene("moved", m, after, prm, total_mass=1000.0, frames=2))
+    second.x = first.x
+    second.v = first.v
+    second.rho_hat = first.rho_hat
+    for _ in range(2):
+        solver.solve()
+        second.frame(1)
+    check_positions(scene, solver, second, tol=1e-5)
+    # the raised sphere pushed the box's bottom centre up
+    x = solver.mesh.verts.x.to_numpy()
+    c = np.linalg.norm(x - np.array([0.5, 0.0, 0.5], dtype=np.float32), axis=1)
+    assert c.min() > 0.2 - 0.02
