@@ -75,6 +75,8 @@ typedef struct xpbd_sdf {
   const float* points; /* hang: n_points x 3 sphere centres, copied by the library */
   int32_t n_points;    /* hang: 1 .. XPBD_MAX_HANG_POINTS */
   int32_t pad;
+  double vel[3];       /* the obstacle's velocity, SdfModel.vel (sdf.py:9); check() treats a
+                          vertex as inside when n . (vel - v) > 0 (sdf.py:20-24) */
 } xpbd_sdf;
 
 /* Constructor arguments of XPBDNeoHookeanSolver (neohookean_solver.py:18-38)
@@ -116,7 +118,7 @@ typedef struct xpbd_solver xpbd_solver;
 
 /* Version of this ABI and the sizes of its structs as compiled, so that a
  * binding can check its own struct layout. */
-#define XPBD_ABI_VERSION 3
+#define XPBD_ABI_VERSION 4
 int xpbd_abi_info(int32_t* version, size_t* desc_size, size_t* sdf_size);
 
 /* Fills *d with the reference defaults (everything but the mesh). */
@@ -152,6 +154,12 @@ int xpbd_get_velocities(xpbd_solver* s, float* xyz);
 int xpbd_set_velocities(xpbd_solver* s, const float* xyz);
 int xpbd_get_predicted(xpbd_solver* s, float* xyz);
 int xpbd_get_masses(xpbd_solver* s, float* m, float* inv_m);
+/* The tets whose volume was at most EPSILON when the masses were last
+ * computed (at creation and by xpbd_compute_lumped_mass): their number and
+ * the first one in the caller's tet order (-1 when there is none).  The
+ * reference prints "Warning: Cell <id> has negative or zero volume" for each
+ * of them (xpbd_base.py:104-106) and goes on; so does this library. */
+int xpbd_get_degenerate_tets(xpbd_solver* s, int32_t* count, int32_t* first);
 /* mesh.cells.lambda_D / lambda_H (neohookean_solver.py:87-92), caller tet order */
 int xpbd_get_multipliers(xpbd_solver* s, float* lambda_d, float* lambda_h);
 /* indices field (xpbd_base.py:57-58, 78-87): n_tets*12 uint32 face indices */

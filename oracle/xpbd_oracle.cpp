@@ -25,7 +25,21 @@
 // omega, time) are doubles here for both R.
 //
 // Built with -ffp-contract=off so that no fused multiply-add changes the op
-// order written below.
+// order written below (liboracle_xpbd.so).  The same source is also built with
+// contraction on (-ffp-contract=fast -mfma, liboracle_xpbd_fma.so): that
+// flavour, and the run-time variants below, exist only to measure how far an
+// f32 run moves when its arithmetic changes in ways the reference leaves
+// open, so that the tolerance gates can be set at that level.
+//
+// Variants (Params.variant, bit flags; 0 is the restatement itself):
+//   kVarOrder  every scatter and every sum is taken in a seeded random order
+//              (the reference's float atomics fix no order)
+//   kVarUlp    1/x and sqrt are perturbed by a seeded -1, 0 or +1 ulp where a
+//              GPU would use its native 1-ulp instructions (rest-shape
+//              inverse, Jacobi average, residual), and x / c is formed as
+//              x * (1/c) there
+//   kVarFold   the compliance is formed as (6 / (lame dt^2)) * (1 / det Dm)
+//              instead of 1 / (lame V0) / dt^2
 //
 // Pins.  Every function is checked against closed forms, brute force or
 // library routines by tests/test_oracle_pins.py (Lame constants, rest shape,
@@ -41,6 +55,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <random>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -49,6 +64,27 @@
 namespace {
 
 constexpr double kEpsilon = 1e-12;  // xpbd_gpu/constants.py:3
+
+constexpr int kVarOrder = 1;
+constexpr int kVarUlp = 2;
+constexpr int kVarFold = 4;
+
+// A seeded choice of -1, 0 or +1 ulp for the value at (a, b).
+inline int ulp_step(uint64_t seed, uint64_t a, uint64_t b) {
+  uint64_t h = seed * 0x9E3779B97F4A7C15ull ^ (a + 0x632BE59BD9B4E019ull) * 0xBF58476D1CE4E5B9ull ^
+               (b + 0x94D049BB133111EBull) * 0x94D049BB133111EBull;
+  h ^= h >> 31;
+  h *= 0xD6E8FEB86659FD93ull;
+  h ^= h >> 32;
+  return int(h % 3) - 1;
+}
+
+// x moved by `step` ulps (f32 only; the f64 run is never perturbed).
+template <typename R>
+inline R nudge(R x, int step) {
+  if (sizeof(R) != sizeof(float) || step == 0) return x;
+  return R(std::nextafter(float(x), step > 0 ? INFINITY : -INFINITY));
+}
 
 template <typename R>
 struct M3 {  // row-major 3x3
@@ -78,11 +114,12 @@ inline R det3(const M3<R>& a) {
          m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
 }
 
-// 3x3 inverse as adjugate / determinant (Matrix.inverse for n == 3).
+// 3x3 inverse as adjugate / determinant (Matrix.inverse for n == 3); `step`
+// moves the reciprocal of the determinant by that many ulps (kVarUlp).
 template <typename R>
-inline M3<R> inv3(const M3<R>& a) {
+inline M3<R> inv3(const M3<R>& a, int step = 0) {
   const auto& m = a.m;
-  const R inv_det = R(1) / det3(a);
+  const R inv_det = nudge(R(1) / det3(a), step);
   M3<R> r;
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j) {
@@ -192,12 +229,156 @@ inline int solve2x2_pivot_lu(R a00, R a01, R a10, R a11, R b0, R b1, R eps, R* x
   return ok;
 }
 
+// The compliances alpha~_D = 1 / (lame_D V0) / dt^2 and alpha~_H (with
+// lame_D = mu = lame2, lame_H = lambda = lame1), neohookean_solver.py:413-419
+// (also :178, :238, :318-319); kVarFold forms them as
+// (6 / (lame dt^2)) * (1 / det Dm) instead.
+template <typename R>
+inline void compliance(R lame1, R lame2, R V0, R inv_det, R dt, int variant, R* aD, R* aH) {
+  if (variant & kVarFold) {
+    *aD = R(6.0 / (double(lame2) * double(dt) * double(dt))) * inv_det;
+    *aH = R(6.0 / (double(lame1) * double(dt) * double(dt))) * inv_det;
+  } else {
+    const R alpha_D = R(1) / (lame2 * V0);
+    const R alpha_H = R(1) / (lame1 * V0);
+    *aD = alpha_D / (dt * dt);
+    *aH = alpha_H / (dt * dt);
+  }
+}
+
+// Block projection of one tet, solve_neohookean_block
+// (neohookean_solver.py:358-451): C_D = |F|^2 - 3, C_H = det F - 1, their
+// vertex gradients, the 2x2 system with the compliances on its diagonal,
+// the pivoted LU solve and its diagonal fallback (:423-432), the multiplier
+// update and the four corner corrections w_i (g_Di dlD + g_Hi dlH).
+// grads (optional, 24): g_D then g_H per vertex.  aux (optional, 8): C_D,
+// C_H, A11, A12, A22, dlD, dlH, ok.
+template <typename R>
+inline void block_projection(const R* const p[4], const R w[4], const M3<R>& idm, R aD, R aH,
+                             R& lam_d, R& lam_h, R* corner, R* grads, R* aux) {
+  const M3<R> F = deformation_gradient(p[0], p[1], p[2], p[3], idm);
+  const R C_D = norm_sqr(F) - R(3);
+  const R C_H = det3(F) - R(1);
+  M3<R> gfd;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * R(2);
+  R gD[4][3], gH[4][3];
+  vertex_grads(gfd, idm, gD);
+  vertex_grads(ddet_dF(F), idm, gH);
+  R A11 = w[0] * dot3(gD[0], gD[0]) + w[1] * dot3(gD[1], gD[1]) +
+          w[2] * dot3(gD[2], gD[2]) + w[3] * dot3(gD[3], gD[3]);
+  R A22 = w[0] * dot3(gH[0], gH[0]) + w[1] * dot3(gH[1], gH[1]) +
+          w[2] * dot3(gH[2], gH[2]) + w[3] * dot3(gH[3], gH[3]);
+  const R A12 = w[0] * dot3(gD[0], gH[0]) + w[1] * dot3(gD[1], gH[1]) +
+                w[2] * dot3(gD[2], gH[2]) + w[3] * dot3(gD[3], gH[3]);
+  A11 += aD;
+  A22 += aH;
+  const R b1 = -(C_D + aD * lam_d);
+  const R b2 = -(C_H + aH * lam_h);
+  const R eps = R(kEpsilon);
+  R dlD, dlH;
+  const int ok = solve2x2_pivot_lu(A11, A12, A12, A22, b1, b2, eps, &dlD, &dlH);
+  if (ok == 0) {  // neohookean_solver.py:425-432
+    dlD = R(0);
+    dlH = R(0);
+    if (std::fabs(A11) > eps) dlD = b1 / A11;
+    if (std::fabs(A22) > eps) dlH = b2 / A22;
+  }
+  lam_d += dlD;
+  lam_h += dlH;
+  for (int k = 0; k < 4; ++k)
+    for (int r = 0; r < 3; ++r) corner[3 * k + r] = w[k] * (gD[k][r] * dlD + gH[k][r] * dlH);
+  if (grads)
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) {
+        grads[3 * k + r] = gD[k][r];
+        grads[12 + 3 * k + r] = gH[k][r];
+      }
+  if (aux) {
+    aux[0] = C_D;
+    aux[1] = C_H;
+    aux[2] = A11;
+    aux[3] = A12;
+    aux[4] = A22;
+    aux[5] = dlD;
+    aux[6] = dlH;
+    aux[7] = R(ok);
+  }
+}
+
+// One scalar constraint of split mode on one tet: solve_hydrostatic
+// (neohookean_solver.py:196-254) when hydro, solve_deviatoric (:144-194)
+// otherwise.  C and w_sum = sum_i w_i |g_i|^2 go to info[0..1] (optional).
+// When w_sum > EPSILON: dlambda = -(C + alpha~ lambda) / (w_sum + alpha~),
+// times the relaxation, lambda += dlambda, corner i = dlambda w_i g_i, and 1
+// (the cons_num increment) is returned; otherwise nothing changes and 0.
+template <typename R>
+inline int split_projection(const R* const p[4], const R w[4], const M3<R>& idm, bool hydro,
+                            R alpha_tilde, R relax, R& lam, R* corner, R* info) {
+  const M3<R> F = deformation_gradient(p[0], p[1], p[2], p[3], idm);
+  R C;
+  M3<R> gf;
+  if (hydro) {
+    C = det3(F) - R(1);
+    gf = ddet_dF(F);
+  } else {
+    C = norm_sqr(F) - R(3);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) gf.m[i][j] = R(2) * F.m[i][j];
+  }
+  R g[4][3];
+  vertex_grads(gf, idm, g);
+  R w_sum = w[0] * dot3(g[0], g[0]);
+  w_sum += w[1] * dot3(g[1], g[1]);
+  w_sum += w[2] * dot3(g[2], g[2]);
+  w_sum += w[3] * dot3(g[3], g[3]);
+  if (info) {
+    info[0] = C;
+    info[1] = w_sum;
+  }
+  if (!(w_sum > R(kEpsilon))) return 0;
+  R d_lambda = -(C + alpha_tilde * lam) / (w_sum + alpha_tilde);
+  d_lambda *= relax;
+  lam += d_lambda;
+  for (int k = 0; k < 4; ++k)
+    for (int r = 0; r < 3; ++r) corner[3 * k + r] = d_lambda * w[k] * g[k][r];
+  return 1;
+}
+
+// compute_xpbd_residual for one tet (neohookean_solver.py:298-329):
+// r = sqrt((C_D + alpha~_D lambda_D)^2 + (C_H + alpha~_H lambda_H)^2); C_D
+// and C_H (the monitor kernels' quantities, :256-296) go to *C_D / *C_H.
+// `step` moves the square root by that many ulps (kVarUlp).
+template <typename R>
+inline R residual_value(const R* const x[4], const M3<R>& idm, R aD, R aH, R lam_d, R lam_h,
+                        R* C_D, R* C_H, int step = 0) {
+  const M3<R> F = deformation_gradient(x[0], x[1], x[2], x[3], idm);
+  *C_D = norm_sqr(F) - R(3);
+  *C_H = det3(F) - R(1);
+  const R rD = *C_D + aD * lam_d;
+  const R rH = *C_H + aH * lam_h;
+  return nudge(std::sqrt(rD * rD + rH * rH), step);
+}
+
+// _chebyshev_update (neohookean_solver.py:131-137) on n scalars: with x the
+// Jacobi iterate and curr the iterate it started from, x_new = gamma (x -
+// curr) + curr, then x = prev + omega (x_new - prev).
+template <typename R>
+inline void chebyshev_blend_range(R* x, const R* curr, const R* prev, size_t lo, size_t hi,
+                                  R omega, R gamma) {
+  for (size_t i = lo; i < hi; ++i) {
+    const R x_new = gamma * (x[i] - curr[i]) + curr[i];
+    x[i] = prev[i] + omega * (x_new - prev[i]);
+  }
+}
+
 struct SdfDesc {
   int kind;        // 0 plane, 1 sphere, 2 cube, 3 hang
   int fixed;
   double p[4];     // plane: point xyz | sphere: center xyz, radius | cube: center | hang: p[3] radius
   double dir[3];   // plane: unit normal (normalized in double at construction, sdf.py:66) |
                    // cube: edge lengths
+  double vel[3];   // the obstacle's velocity, SdfModel.vel (sdf.py:9)
 };
 
 // One collider as the reference's f32 fields hold it (sdf.py:6-124).  Taichi's
@@ -205,12 +386,13 @@ struct SdfDesc {
 template <typename R>
 struct Sdf {
   int kind, fixed;
-  R p[3], r, d[3];
+  R p[3], r, d[3], vel[3];
   std::vector<R> pts;  // hang: sphere centres, xyz per sphere
   explicit Sdf(const SdfDesc& s) : kind(s.kind), fixed(s.fixed) {
     for (int i = 0; i < 3; ++i) {
       p[i] = R(s.p[i]);
       d[i] = kind == 2 ? R(s.dir[i] * 0.5) : R(s.dir[i]);  // cube: half_size = size * 0.5
+      vel[i] = R(s.vel[i]);
     }
     r = R(s.p[3]);
     if (kind == 3) fixed = 1;  // HangSdfModel(fixed=True), sdf.py:31
@@ -270,15 +452,15 @@ struct Sdf {
     if (std::fabs(q[2]) > std::fabs(q[axis])) axis = 2;
     n[axis] = q[axis] > R(0) ? R(1) : R(-1);
   }
-  // SdfModel.check (sdf.py:13-27): returns inside, fills n.  The obstacle's
-  // own velocity is always zero in the reference (never written).
+  // SdfModel.check (sdf.py:13-27): returns inside, fills n.  diff_vel is the
+  // obstacle's velocity minus the vertex's (zero unless the caller sets it).
   bool check(const R* x, const R* v, R* n) const {
     const R phi = dist(x);
     bool inside = false;
     n[0] = n[1] = n[2] = R(0);
     if (phi < R(0)) {
       normal(x, n);
-      const R dv[3] = {R(0) - v[0], R(0) - v[1], R(0) - v[2]};
+      const R dv[3] = {vel[0] - v[0], vel[1] - v[1], vel[2] - v[2]};
       const R dotnv = dot3(n, dv);
       if (dotnv > R(0) || fixed) inside = true;
     }
@@ -295,6 +477,8 @@ struct Params {
   double cheb_gamma;
   int block_enable, monitor;
   double damping, gravity[3], total_mass;
+  int variant;    // kVar* flags; 0 for the restatement itself
+  uint64_t seed;  // of the variant's random orders and ulp steps
 };
 
 struct IterLog {
@@ -321,6 +505,7 @@ class Solver {
     prev_.assign(3 * size_t(nv), R(0));
     curr_.assign(3 * size_t(nv), R(0));
     V0_.assign(nt, R(0));
+    inv_det_.assign(nt, R(0));
     inv_dm_.resize(nt);
     lam_h_.assign(nt, R(0));
     lam_d_.assign(nt, R(0));
@@ -330,6 +515,14 @@ class Solver {
     per_tet_.assign(nt, R(0));
     per_tet2_.assign(nt, R(0));
     per_tet3_.assign(nt, R(0));
+    variant_ = prm.variant;
+    seed_ = prm.seed;
+    sum_order_.resize(nt);
+    for (int t = 0; t < nt; ++t) sum_order_[t] = t;
+    if (variant_ & kVarOrder) {
+      std::mt19937_64 rng(seed_ * 2654435761ull + 17);
+      std::shuffle(sum_order_.begin(), sum_order_.end(), rng);
+    }
     build_adjacency();
     set_lame();
     gravity_[0] = R(prm_.gravity[0]);
@@ -392,6 +585,33 @@ class Solver {
     for (size_t k = 0; k < 3 * size_t(n); ++k) sdfs_[i].pts[k] = R(float(xyz[k]));
   }
   void set_rho_hat(double v) { rho_hat_ = v; }
+  void set_pred(const double* in) {
+    for (size_t i = 0; i < pred_.size(); ++i) pred_[i] = R(in[i]);
+  }
+  void set_lambda(const double* ld, const double* lh) {
+    for (int t = 0; t < nt_; ++t) {
+      lam_d_[t] = R(ld[t]);
+      lam_h_[t] = R(lh[t]);
+    }
+  }
+  // Residual sums on the current pred_ and multipliers with time step dt:
+  // out = sum r, sum |C_H|, sum |C_D| (the quantities of
+  // neohookean_solver.py:298-329 and :256-296).
+  void residual_now(double dt, double* out) {
+    R hs, ds;
+    const R rs = residual(R(dt), &hs, &ds);
+    out[0] = double(rs);
+    out[1] = double(hs);
+    out[2] = double(ds);
+  }
+  // Per-iteration trace of the Chebyshev sequence (first substep after it is
+  // enabled): per iteration the iterate it started from (curr), the Jacobi
+  // iterate, the iterate after the blend, and omega.
+  void set_trace(int on) {
+    trace_on_ = on != 0;
+    trace_.clear();
+  }
+  const std::vector<double>& trace() const { return trace_; }
   double time() const { return time_; }
   int frame() const { return frame_; }
   const std::vector<IterLog>& log() const { return log_; }
@@ -425,7 +645,9 @@ class Solver {
         dm.m[r][1] = p[2][r] - p[0][r];
         dm.m[r][2] = p[3][r] - p[0][r];
       }
-      inv_dm_[t] = inv3(dm);
+      const int step = (variant_ & kVarUlp) ? ulp_step(seed_, uint64_t(t), 0x1d) : 0;
+      inv_dm_[t] = inv3(dm, step);
+      inv_det_[t] = nudge(R(1) / det3(dm), step);
     }
   }
 
@@ -448,7 +670,8 @@ class Solver {
     }
     R total_volume = R(0);
     for (int v = 0; v < nv_; ++v) m_[v] = R(0);
-    for (int t = 0; t < nt_; ++t) {
+    for (int i = 0; i < nt_; ++i) {
+      const int t = sum_order_[i];
       const R quarter = per_tet_[t] * R(0.25);
       for (int k = 0; k < 4; ++k) m_[tets_[4 * size_t(t) + k]] += quarter;
       total_volume += per_tet_[t];
@@ -519,8 +742,8 @@ class Solver {
   // ---- Jacobi iteration kernels ----------------------------------------------
 
   // Per-tet block projection of solve_neohookean_block
-  // (neohookean_solver.py:358-451).  Writes the 4 corner corrections into
-  // `corner` (12 values) and updates the multipliers.
+  // (neohookean_solver.py:358-451): the 4 corner corrections go into `corner`
+  // (12 values) and the multipliers are updated.
   void project_block_tet(int t, R dt, R* corner) {
     const int* tv = &tets_[4 * size_t(t)];
     const R* p[4];
@@ -529,43 +752,10 @@ class Solver {
       p[k] = &pred_[3 * size_t(tv[k])];
       w[k] = inv_m_[tv[k]];
     }
-    const M3<R>& idm = inv_dm_[t];
-    const M3<R> F = deformation_gradient(p[0], p[1], p[2], p[3], idm);
-    const R C_D = norm_sqr(F) - R(3);
-    const R C_H = det3(F) - R(1);
-    M3<R> gfd;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * R(2);
-    R gD[4][3], gH[4][3];
-    vertex_grads(gfd, idm, gD);
-    vertex_grads(ddet_dF(F), idm, gH);
-    R A11 = w[0] * dot3(gD[0], gD[0]) + w[1] * dot3(gD[1], gD[1]) +
-            w[2] * dot3(gD[2], gD[2]) + w[3] * dot3(gD[3], gD[3]);
-    R A22 = w[0] * dot3(gH[0], gH[0]) + w[1] * dot3(gH[1], gH[1]) +
-            w[2] * dot3(gH[2], gH[2]) + w[3] * dot3(gH[3], gH[3]);
-    const R A12 = w[0] * dot3(gD[0], gH[0]) + w[1] * dot3(gD[1], gH[1]) +
-                  w[2] * dot3(gD[2], gH[2]) + w[3] * dot3(gD[3], gH[3]);
-    const R alpha_D = R(1) / (lame2_ * V0_[t]);
-    const R alpha_H = R(1) / (lame1_ * V0_[t]);
-    const R aD = alpha_D / (dt * dt);
-    const R aH = alpha_H / (dt * dt);
-    A11 += aD;
-    A22 += aH;
-    const R b1 = -(C_D + aD * lam_d_[t]);
-    const R b2 = -(C_H + aH * lam_h_[t]);
-    const R eps = R(kEpsilon);
-    R dlD, dlH;
-    const int ok = solve2x2_pivot_lu(A11, A12, A12, A22, b1, b2, eps, &dlD, &dlH);
-    if (ok == 0) {  // neohookean_solver.py:425-432
-      dlD = R(0);
-      dlH = R(0);
-      if (std::fabs(A11) > eps) dlD = b1 / A11;
-      if (std::fabs(A22) > eps) dlH = b2 / A22;
-    }
-    lam_d_[t] += dlD;
-    lam_h_[t] += dlH;
-    for (int k = 0; k < 4; ++k)
-      for (int r = 0; r < 3; ++r) corner[3 * k + r] = w[k] * (gD[k][r] * dlD + gH[k][r] * dlH);
+    R aD, aH;
+    compliance(lame1_, lame2_, V0_[t], inv_det_[t], dt, variant_, &aD, &aH);
+    block_projection(p, w, inv_dm_[t], aD, aH, lam_d_[t], lam_h_[t], corner, (R*)nullptr,
+                     (R*)nullptr);
   }
 
   // One scalar constraint of split mode: solve_hydrostatic
@@ -579,35 +769,11 @@ class Solver {
       p[k] = &pred_[3 * size_t(tv[k])];
       w[k] = inv_m_[tv[k]];
     }
-    const M3<R>& idm = inv_dm_[t];
-    const M3<R> F = deformation_gradient(p[0], p[1], p[2], p[3], idm);
-    R C;
-    M3<R> gf;
-    if (hydro) {
-      C = det3(F) - R(1);
-      gf = ddet_dF(F);
-    } else {
-      C = norm_sqr(F) - R(3);
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) gf.m[i][j] = R(2) * F.m[i][j];
-    }
-    R g[4][3];
-    vertex_grads(gf, idm, g);
-    R w_sum = w[0] * dot3(g[0], g[0]);
-    w_sum += w[1] * dot3(g[1], g[1]);
-    w_sum += w[2] * dot3(g[2], g[2]);
-    w_sum += w[3] * dot3(g[3], g[3]);
-    if (!(w_sum > R(kEpsilon))) return 0;
-    const R lame = hydro ? lame1_ : lame2_;
-    const R alpha = R(1) / (lame * V0_[t]);
-    const R alpha_tilde = alpha / (dt * dt);
-    R& lam = hydro ? lam_h_[t] : lam_d_[t];
-    R d_lambda = -(C + alpha_tilde * lam) / (w_sum + alpha_tilde);
-    d_lambda *= R(hydro ? prm_.hydro_relax : prm_.dev_relax);
-    lam += d_lambda;
-    for (int k = 0; k < 4; ++k)
-      for (int r = 0; r < 3; ++r) corner[3 * k + r] = d_lambda * w[k] * g[k][r];
-    return 1;
+    R aD, aH;
+    compliance(lame1_, lame2_, V0_[t], inv_det_[t], dt, variant_, &aD, &aH);
+    return split_projection(p, w, inv_dm_[t], hydro, hydro ? aH : aD,
+                            R(hydro ? prm_.hydro_relax : prm_.dev_relax),
+                            hydro ? lam_h_[t] : lam_d_[t], corner, (R*)nullptr);
   }
 
   // pre_solve + projection + post_solve (neohookean_solver.py:109-119, 482-490).
@@ -653,9 +819,16 @@ class Solver {
       }
       for (int k = 0; k < 3; ++k) dx_[3 * v + k] = d[k];
       cons_[v] = cnt;
-      if (cnt > 0)
-        for (int k = 0; k < 3; ++k) pred_[3 * v + k] += d[k] / R(cnt);
+      if (cnt > 0) {
+        if (variant_ & kVarUlp) {
+          const R ic = nudge(R(1) / R(cnt), ulp_step(seed_, uint64_t(v), 2 * iter_count_ + 1));
+          for (int k = 0; k < 3; ++k) pred_[3 * v + k] += d[k] * ic;
+        } else {
+          for (int k = 0; k < 3; ++k) pred_[3 * v + k] += d[k] / R(cnt);
+        }
+      }
     }
+    ++iter_count_;
   }
 
   // compute_xpbd_residual (neohookean_solver.py:298-329), plus the monitor
@@ -666,23 +839,19 @@ class Solver {
 #pragma omp parallel for num_threads(threads_) schedule(static)
     for (int t = 0; t < nt_; ++t) {
       const int* tv = &tets_[4 * size_t(t)];
-      const M3<R> F = deformation_gradient(&pred_[3 * size_t(tv[0])], &pred_[3 * size_t(tv[1])],
-                                           &pred_[3 * size_t(tv[2])], &pred_[3 * size_t(tv[3])],
-                                           inv_dm_[t]);
-      const R C_D = norm_sqr(F) - R(3);
-      const R C_H = det3(F) - R(1);
-      const R alpha_D = R(1) / (lame2_ * V0_[t]);
-      const R alpha_H = R(1) / (lame1_ * V0_[t]);
-      const R aD = alpha_D / (dt * dt);
-      const R aH = alpha_H / (dt * dt);
-      const R rD = C_D + aD * lam_d_[t];
-      const R rH = C_H + aH * lam_h_[t];
-      per_tet_[t] = std::sqrt(rD * rD + rH * rH);
+      const R* x[4] = {&pred_[3 * size_t(tv[0])], &pred_[3 * size_t(tv[1])],
+                       &pred_[3 * size_t(tv[2])], &pred_[3 * size_t(tv[3])]};
+      R aD, aH, C_D, C_H;
+      compliance(lame1_, lame2_, V0_[t], inv_det_[t], dt, variant_, &aD, &aH);
+      const int step =
+          (variant_ & kVarUlp) ? ulp_step(seed_, uint64_t(t), 2 * iter_count_ + 2) : 0;
+      per_tet_[t] = residual_value(x, inv_dm_[t], aD, aH, lam_d_[t], lam_h_[t], &C_D, &C_H, step);
       per_tet2_[t] = C_H;
       per_tet3_[t] = C_D;
     }
     R s = R(0), sh = R(0), sd = R(0);
-    for (int t = 0; t < nt_; ++t) {
+    for (int i = 0; i < nt_; ++i) {
+      const int t = sum_order_[i];
       s += per_tet_[t];
       sh += std::fabs(per_tet2_[t]);
       sd += std::fabs(per_tet3_[t]);
@@ -704,9 +873,12 @@ class Solver {
     double omega = 1.0;
     if (use_cheb) prev_ = pred_;  // _save_curr_to_prev
     const R gamma = R(prm_.cheb_gamma);
+    const bool tracing = trace_on_ && trace_.empty();
     for (int it = 0; it < prm_.iters; ++it) {
       if (use_cheb) curr_ = pred_;  // _save_curr_to_savebuf
+      if (tracing) trace_.insert(trace_.end(), pred_.begin(), pred_.end());
       jacobi_iteration(dt);
+      if (tracing) trace_.insert(trace_.end(), pred_.begin(), pred_.end());
       double r_hat = 0.0;
       R hs = R(0), ds = R(0);
       const bool need_res = use_cheb || prm_.monitor;
@@ -735,6 +907,10 @@ class Solver {
         chebyshev_update(R(omega), gamma);
         prev_ = curr_;  // _rotate_prev_with_save
       }
+      if (tracing) {
+        trace_.insert(trace_.end(), pred_.begin(), pred_.end());
+        trace_.push_back(use_cheb ? omega : 1.0);
+      }
       IterLog rec;
       rec.frame = frame_;
       rec.substep = substep_;
@@ -751,11 +927,11 @@ class Solver {
 
   // _chebyshev_update (neohookean_solver.py:131-137)
   void chebyshev_update(R omega, R gamma) {
+    const long n = long(pred_.size());
 #pragma omp parallel for num_threads(threads_) schedule(static)
-    for (size_t i = 0; i < pred_.size(); ++i) {
-      const R x_new = gamma * (pred_[i] - curr_[i]) + curr_[i];
-      pred_[i] = prev_[i] + omega * (x_new - prev_[i]);
-    }
+    for (long i = 0; i < n; ++i)
+      chebyshev_blend_range(pred_.data(), curr_.data(), prev_.data(), size_t(i), size_t(i) + 1,
+                            omega, gamma);
   }
 
   // solve (neohookean_solver.py:562-587)
@@ -795,6 +971,11 @@ class Solver {
     std::vector<int> fill(adj_off_.begin(), adj_off_.end() - 1);
     for (int t = 0; t < nt_; ++t)  // ascending t -> each list sorted by tet
       for (int k = 0; k < 4; ++k) adj_[fill[tets_[4 * size_t(t) + k]]++] = 4 * t + k;
+    if (variant_ & kVarOrder) {  // each vertex's corrections in a random order
+      std::mt19937_64 rng(seed_ * 0x9E3779B97F4A7C15ull + 5);
+      for (int v = 0; v < nv_; ++v)
+        std::shuffle(adj_.begin() + adj_off_[v], adj_.begin() + adj_off_[v + 1], rng);
+    }
   }
 
   int nv_, nt_;
@@ -804,8 +985,14 @@ class Solver {
   std::vector<Sdf<R>> sdfs_;
   std::vector<R> x_, pred_, v_, m_, inv_m_, dx_, prev_, curr_;
   std::vector<uint32_t> cons_;
-  std::vector<R> V0_, lam_h_, lam_d_;
+  std::vector<R> V0_, inv_det_, lam_h_, lam_d_;
   std::vector<M3<R>> inv_dm_;
+  int variant_ = 0;
+  uint64_t seed_ = 0;
+  uint64_t iter_count_ = 0;     // Jacobi iterations run (keys the ulp steps)
+  std::vector<int> sum_order_;  // order of the tet sums (a permutation under kVarOrder)
+  bool trace_on_ = false;
+  std::vector<double> trace_;
   std::vector<R> corner_a_, corner_b_, per_tet_, per_tet2_, per_tet3_;
   std::vector<int> corner_ok_;
   std::vector<int> adj_off_, adj_;
@@ -854,6 +1041,8 @@ Params params_from(const double* p) {
   q.gravity[1] = p[19];
   q.gravity[2] = p[20];
   q.total_mass = p[21];
+  q.variant = int(p[22]);
+  q.seed = uint64_t(p[23]);
   return q;
 }
 
@@ -861,18 +1050,20 @@ Params params_from(const double* p) {
 
 extern "C" {
 
-// params: 22 doubles, in the order read by params_from().
-// sdf_kind[n] (0 plane, 1 sphere, 2 cube, 3 hang), sdf_fixed[n], sdf_p[4n], sdf_dir[3n];
-// a hang collider's centres follow with oracle_set_sdf_points.
+// params: 24 doubles, in the order read by params_from().
+// sdf_kind[n] (0 plane, 1 sphere, 2 cube, 3 hang), sdf_fixed[n], sdf_p[4n], sdf_dir[3n],
+// sdf_vel[3n]; a hang collider's centres follow with oracle_set_sdf_points.
 void* oracle_create(int precision, int nv, int nt, const float* rest, const int* tets,
                     const double* params, int n_sdf, const int* sdf_kind, const int* sdf_fixed,
-                    const double* sdf_p, const double* sdf_dir, int threads) {
+                    const double* sdf_p, const double* sdf_dir, const double* sdf_vel,
+                    int threads) {
   std::vector<SdfDesc> sd(n_sdf);
   for (int i = 0; i < n_sdf; ++i) {
     sd[i].kind = sdf_kind[i];
     sd[i].fixed = sdf_fixed[i];
     for (int k = 0; k < 4; ++k) sd[i].p[k] = sdf_p[4 * i + k];
     for (int k = 0; k < 3; ++k) sd[i].dir[k] = sdf_dir[3 * i + k];
+    for (int k = 0; k < 3; ++k) sd[i].vel[k] = sdf_vel[3 * i + k];
   }
   const Params prm = params_from(params);
   Handle* h = new Handle;
@@ -977,66 +1168,119 @@ void oracle_clear_log(void* hp) {
   dispatch(static_cast<Handle*>(hp), [](auto& s) { s.clear_log(); return 0; });
 }
 
-// ---- single-tet entry points (for closed-form and finite-difference pins) ----
+// ---- state access for the pins ------------------------------------------------
 
-// Block projection of one tet in isolation.  p: 4x3 positions, w: 4 inverse
-// masses, inv_dm: 3x3 row-major, lam: [lambda_D, lambda_H] in/out, out
-// corner: 4x3 corrections.  aux (8): C_D, C_H, A11, A12, A22, dlamD, dlamH, ok.
+void oracle_set_pred(void* hp, const double* in) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_pred(in); return 0; });
+}
+void oracle_set_lambda(void* hp, const double* lam_d, const double* lam_h) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_lambda(lam_d, lam_h); return 0; });
+}
+// sum r, sum |C_H|, sum |C_D| over the tets at the current pred_x and
+// multipliers, with time step dt
+void oracle_residual_now(void* hp, double dt, double* out) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.residual_now(dt, out); return 0; });
+}
+// Trace of the first substep's Chebyshev sequence after enabling: per
+// iteration curr (3V), Jacobi iterate (3V), blended iterate (3V), omega.
+void oracle_set_trace(void* hp, int on) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) { s.set_trace(on); return 0; });
+}
+int oracle_trace_size(void* hp) {
+  return dispatch(static_cast<Handle*>(hp), [](auto& s) { return int(s.trace().size()); });
+}
+void oracle_get_trace(void* hp, double* out) {
+  dispatch(static_cast<Handle*>(hp), [&](auto& s) {
+    std::copy(s.trace().begin(), s.trace().end(), out);
+    return 0;
+  });
+}
+
+// ---- single-tet entry points (for closed-form and finite-difference pins) ----
+// They call the same per-tet functions as the solver above.
+
+}  // extern "C"
+
+namespace {
+template <typename R>
+M3<R> load_m3(const double* a) {
+  M3<R> m;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) m.m[i][j] = R(a[3 * i + j]);
+  return m;
+}
+
+template <typename R>
+void block_tet(const double* p, const double* w, const double* inv_dm, double v0, double lame1,
+               double lame2, double dt, double* lam, double* corner, double* grads, double* aux) {
+  R pr[12], wr[4], c[12], g[24], a[8];
+  for (int i = 0; i < 12; ++i) pr[i] = R(p[i]);
+  for (int i = 0; i < 4; ++i) wr[i] = R(w[i]);
+  const R* pp[4] = {&pr[0], &pr[3], &pr[6], &pr[9]};
+  const M3<R> idm = load_m3<R>(inv_dm);
+  R aD, aH;
+  compliance(R(lame1), R(lame2), R(v0), R(0), R(dt), 0, &aD, &aH);
+  R ld = R(lam[0]), lh = R(lam[1]);
+  block_projection(pp, wr, idm, aD, aH, ld, lh, c, g, a);
+  lam[0] = double(ld);
+  lam[1] = double(lh);
+  for (int i = 0; i < 12; ++i) corner[i] = double(c[i]);
+  if (grads)
+    for (int i = 0; i < 24; ++i) grads[i] = double(g[i]);
+  if (aux)
+    for (int i = 0; i < 8; ++i) aux[i] = double(a[i]);
+}
+}  // namespace
+
+extern "C" {
+
+// Block projection of one tet in isolation (precision 32 or 64).  p: 4x3
+// positions, w: 4 inverse masses, inv_dm: 3x3 row-major, lam: [lambda_D,
+// lambda_H] in/out, corner: 4x3 corrections, grads (24, optional): g_D then
+// g_H per vertex; aux (8): C_D, C_H, A11, A12, A22, dlamD, dlamH, ok.
+void oracle_block_tet(int precision, const double* p, const double* w, const double* inv_dm,
+                      double v0, double lame1, double lame2, double dt, double* lam,
+                      double* corner, double* grads, double* aux) {
+  if (precision == 32)
+    block_tet<float>(p, w, inv_dm, v0, lame1, lame2, dt, lam, corner, grads, aux);
+  else
+    block_tet<double>(p, w, inv_dm, v0, lame1, lame2, dt, lam, corner, grads, aux);
+}
+
 void oracle_block_tet64(const double* p, const double* w, const double* inv_dm, double v0,
                         double lame1, double lame2, double dt, double* lam, double* corner,
                         double* grads, double* aux) {
-  M3<double> idm;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) idm.m[i][j] = inv_dm[3 * i + j];
-  const M3<double> F = deformation_gradient(&p[0], &p[3], &p[6], &p[9], idm);
-  const double C_D = norm_sqr(F) - 3.0;
-  const double C_H = det3(F) - 1.0;
-  M3<double> gfd;
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) gfd.m[i][j] = F.m[i][j] * 2.0;
-  double gD[4][3], gH[4][3];
-  vertex_grads(gfd, idm, gD);
-  vertex_grads(ddet_dF(F), idm, gH);
-  double A11 = 0, A22 = 0, A12 = 0;
-  for (int k = 0; k < 4; ++k) {
-    A11 += w[k] * dot3(gD[k], gD[k]);
-    A22 += w[k] * dot3(gH[k], gH[k]);
-    A12 += w[k] * dot3(gD[k], gH[k]);
-  }
-  const double aD = (1.0 / (lame2 * v0)) / (dt * dt);
-  const double aH = (1.0 / (lame1 * v0)) / (dt * dt);
-  A11 += aD;
-  A22 += aH;
-  const double b1 = -(C_D + aD * lam[0]);
-  const double b2 = -(C_H + aH * lam[1]);
-  double dlD, dlH;
-  const int ok = solve2x2_pivot_lu(A11, A12, A12, A22, b1, b2, kEpsilon, &dlD, &dlH);
-  if (ok == 0) {
-    dlD = 0.0;
-    dlH = 0.0;
-    if (std::fabs(A11) > kEpsilon) dlD = b1 / A11;
-    if (std::fabs(A22) > kEpsilon) dlH = b2 / A22;
-  }
-  lam[0] += dlD;
-  lam[1] += dlH;
-  for (int k = 0; k < 4; ++k)
-    for (int r = 0; r < 3; ++r) {
-      corner[3 * k + r] = w[k] * (gD[k][r] * dlD + gH[k][r] * dlH);
-      if (grads) {
-        grads[3 * k + r] = gD[k][r];
-        grads[12 + 3 * k + r] = gH[k][r];
-      }
-    }
-  if (aux) {
-    aux[0] = C_D;
-    aux[1] = C_H;
-    aux[2] = A11;
-    aux[3] = A12;
-    aux[4] = A22;
-    aux[5] = dlD;
-    aux[6] = dlH;
-    aux[7] = ok;
-  }
+  block_tet<double>(p, w, inv_dm, v0, lame1, lame2, dt, lam, corner, grads, aux);
+}
+
+// One split-mode constraint of one tet in float64 (hydro != 0: C_H, else
+// C_D) with compliance 1 / (lame V0) / dt^2 and relaxation `relax`.  lam
+// in/out; corner: 4x3; info (2): C, w_sum.  Returns the cons_num increment.
+int oracle_split_tet64(const double* p, const double* w, const double* inv_dm, double v0,
+                       double lame, double dt, double relax, int hydro, double* lam,
+                       double* corner, double* info) {
+  const double* pp[4] = {&p[0], &p[3], &p[6], &p[9]};
+  const M3<double> idm = load_m3<double>(inv_dm);
+  double aD, aH;
+  compliance(lame, lame, v0, 0.0, dt, 0, &aD, &aH);
+  std::fill(corner, corner + 12, 0.0);
+  return split_projection(pp, w, idm, hydro != 0, hydro ? aH : aD, relax, *lam, corner, info);
+}
+
+// Residual of one tet in float64: out (3) = r, C_D, C_H.
+void oracle_residual_tet64(const double* p, const double* inv_dm, double v0, double lame1,
+                           double lame2, double dt, double lam_d, double lam_h, double* out) {
+  const double* pp[4] = {&p[0], &p[3], &p[6], &p[9]};
+  const M3<double> idm = load_m3<double>(inv_dm);
+  double aD, aH;
+  compliance(lame1, lame2, v0, 0.0, dt, 0, &aD, &aH);
+  out[0] = residual_value(pp, idm, aD, aH, lam_d, lam_h, &out[1], &out[2]);
+}
+
+// The Chebyshev blend of n values in float64 (_chebyshev_update).
+void oracle_cheb_blend64(double* x, const double* curr, const double* prev, int n, double omega,
+                         double gamma) {
+  chebyshev_blend_range(x, curr, prev, 0, size_t(n), omega, gamma);
 }
 
 int oracle_solve2x2_64(double a00, double a01, double a10, double a11, double b0, double b1,

@@ -47,6 +47,8 @@ struct SdfDev {
   float d[3];  // plane unit normal | cube half size
   int n_pts;   // hang: number of spheres
   gptr<const float4> pts;  // hang: sphere centres (xyz)
+  float vel[3];  // the obstacle's velocity (SdfModel.vel, sdf.py:9)
+  int pad;
 };
 
 // Per-scene scalar state that persists across launches (rho_hat persists over
@@ -366,15 +368,15 @@ __host__ __device__ inline void sdf_normal(const SdfDev& s, const float x[3], fl
   n[axis] = q[axis] > 0.f ? 1.f : -1.f;
 }
 
-// SdfModel.check (sdf.py:13-27) with the obstacle's velocity, which the
-// reference never sets, at zero: inside when phi < 0 and (n . (0 - v) > 0 or
-// fixed).
+// SdfModel.check (sdf.py:13-27): inside when phi < 0 and (n . (vel_sdf - v) > 0
+// or fixed), with vel_sdf the obstacle's velocity.
 __host__ __device__ inline bool sdf_inside(const SdfDev& s, const float x[3], const float v[3],
                                            float n[3]) {
   n[0] = n[1] = n[2] = 0.f;
   if (!(sdf_dist(s, x) < 0.f)) return false;
   sdf_normal(s, x, n);
-  const float dotnv = n[0] * (0.f - v[0]) + n[1] * (0.f - v[1]) + n[2] * (0.f - v[2]);
+  const float dv0 = s.vel[0] - v[0], dv1 = s.vel[1] - v[1], dv2 = s.vel[2] - v[2];
+  const float dotnv = n[0] * dv0 + n[1] * dv1 + n[2] * dv2;
   return dotnv > 0.f || s.fixed;
 }
 

@@ -146,6 +146,7 @@ SdfDev to_dev(const xpbd_sdf& s, const float4* pts) {
   d.kind = s.kind;
   d.fixed = s.fixed;
   for (int k = 0; k < 3; ++k) d.p[k] = float(s.p[k]);
+  for (int k = 0; k < 3; ++k) d.vel[k] = float(s.vel[k]);
   d.r = float(s.p[3]);
   if (s.kind == XPBD_SDF_PLANE) {
     // PlaneSdf normalizes dir at construction in Python (sdf.py:66).
@@ -416,6 +417,8 @@ struct xpbd_solver {
   std::vector<int> tets_orig;      // caller order, for the visual indices
   std::vector<int> tets_new;       // internal order, internal vertex ids
   std::vector<float> m_host, inv_m_host;  // internal order
+  int n_degenerate = 0;     // tets of volume <= EPSILON at the last mass computation
+  int first_degenerate = -1;  // the first of them in the caller's tet order
 
   DevBuf<float4> pos, xs, vel, prev, xj, dacc, corner, rest, slot_acc, ybuf1, ybuf2;
   bool has_isolated = false;
@@ -697,10 +700,17 @@ void lumped_mass_host(xpbd_solver* s, const std::vector<float>& x, const std::ve
   s->m_host.assign(s->nv, 0.f);
   s->inv_m_host.assign(s->nv, 0.f);
   float total_volume = 0.f;
+  s->n_degenerate = 0;
+  s->first_degenerate = -1;
   for (int t = 0; t < s->nt; ++t) {
     const int* tv = &s->tets_new[4 * size_t(t)];
     const float vol = tet_volume_f32(&x[3 * size_t(tv[0])], &x[3 * size_t(tv[1])],
                                      &x[3 * size_t(tv[2])], &x[3 * size_t(tv[3])]);
+    if (vol <= xpbd::kEps) {  // the reference's warning condition (xpbd_base.py:104-106)
+      const int caller = s->t_new2old[t];
+      s->n_degenerate++;
+      if (s->first_degenerate < 0 || caller < s->first_degenerate) s->first_degenerate = caller;
+    }
     const float quarter = vol * 0.25f;
     for (int k = 0; k < 4; ++k) s->m_host[tv[k]] += quarter;
     total_volume += vol;
@@ -1377,6 +1387,13 @@ int xpbd_get_masses(xpbd_solver* s, float* m, float* inv_m) {
   return XPBD_OK;
 }
 
+int xpbd_get_degenerate_tets(xpbd_solver* s, int32_t* count, int32_t* first) {
+  if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
+  if (count) *count = s->n_degenerate;
+  if (first) *first = s->first_degenerate;
+  return XPBD_OK;
+}
+
 int xpbd_get_multipliers(xpbd_solver* s, float* lambda_d, float* lambda_h) {
   CHECK_HANDLE(s);
   std::vector<float2> l(s->nt);
@@ -1679,7 +1696,31 @@ struct xpbd_batch {
 
 namespace {
 
+// Members of one batch share what the launch sequence depends on: the
+// iteration count, time step, substep count and mode flags (checked on the
+// members' current parameters).
+int batch_uniform_params(xpbd_solver* const* scenes, int n) {
+  const xpbd_solver* s0 = scenes[0];
+  for (int i = 1; i < n; ++i) {
+    const xpbd_solver* s = scenes[i];
+    const xpbd_desc &a = s->prm, &r = s0->prm;
+    if (a.constraints_iter != r.constraints_iter || a.substeps_num != r.substeps_num ||
+        a.frame_dt != r.frame_dt || a.dt != r.dt || a.cheb_enable != r.cheb_enable ||
+        a.block_neohookean_enable != r.block_neohookean_enable ||
+        a.monitor_convergence != r.monitor_convergence || a.accumulate_mode != r.accumulate_mode ||
+        s->has_isolated != s0->has_isolated)
+      return set_error(XPBD_EINVAL, "scene %d: iteration count, time step or mode flags differ", i);
+  }
+  return XPBD_OK;
+}
+
 int batch_sync_table(xpbd_batch* b) {
+  // a member's parameters may have changed since the batch was built (for
+  // example by xpbd_update_params): the members must still agree, and a
+  // change of the shared sequence drops the captured graph
+  if (batch_uniform_params(b->scenes.data(), int(b->scenes.size())) != XPBD_OK)
+    return set_error(XPBD_ESTATE, "batch members no longer share the iteration count, time "
+                                  "step or mode flags: re-target the batch");
   bool changed = b->table_host.size() != b->scenes.size();
   b->table_host.resize(b->scenes.size());
   for (size_t i = 0; i < b->scenes.size(); ++i) {
@@ -1695,6 +1736,16 @@ int batch_sync_table(xpbd_batch* b) {
   HIP_TRY(hipStreamSynchronize(b->stream));
   HIP_TRY(hipMemcpy(b->table.p, b->table_host.data(), sizeof(SceneDev) * b->table_host.size(),
                     hipMemcpyHostToDevice));
+  const SceneDev& p = b->table_host[0];
+  if (p.iters != b->proto.iters || p.fused != b->proto.fused || p.cheb != b->proto.cheb ||
+      p.monitor != b->proto.monitor) {
+    if (b->gexec) {
+      HIP_TRY(hipGraphExecDestroy(b->gexec));
+      b->gexec = nullptr;
+      b->graph_substeps = 0;
+    }
+  }
+  b->proto = p;
   return XPBD_OK;
 }
 
@@ -1712,14 +1763,8 @@ int batch_check_members(xpbd_solver* const* scenes, int n, int device) {
     if (s->device != device) return set_error(XPBD_EINVAL, "scene %d on another device", i);
     if (s->layout != XPBD_LAYOUT_PATCH)
       return set_error(XPBD_EINVAL, "scene %d: batches need the patch layout", i);
-    const xpbd_desc &a = s->prm, &r = s0->prm;
-    if (a.constraints_iter != r.constraints_iter || a.substeps_num != r.substeps_num ||
-        a.frame_dt != r.frame_dt || a.dt != r.dt || a.cheb_enable != r.cheb_enable ||
-        a.block_neohookean_enable != r.block_neohookean_enable ||
-        a.monitor_convergence != r.monitor_convergence || a.accumulate_mode != r.accumulate_mode ||
-        s->has_isolated != s0->has_isolated)
-      return set_error(XPBD_EINVAL, "scene %d: iteration count, time step or mode flags differ", i);
   }
+  TRY(batch_uniform_params(scenes, n));
   for (int i = 0; i < n; ++i) HIP_TRY(hipStreamSynchronize(scenes[i]->stream));
   return XPBD_OK;
 }
@@ -1766,8 +1811,8 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes) {
   if (b->table.cap < size_t(n_scenes)) TRY(b->table.alloc(n_scenes));
   b->table.n = size_t(n_scenes);
   b->table_host.clear();  // forces the upload
+  b->proto = b->scenes[0]->host_scene;
   TRY(batch_sync_table(b));
-  b->proto = b->table_host[0];
   return XPBD_OK;
 }
 

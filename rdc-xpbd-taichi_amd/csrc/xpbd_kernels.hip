@@ -402,6 +402,9 @@ constexpr int kReduceThreads = kPatchThreads;
 
 // LDS doubles a scene reduction needs as scratch.
 constexpr int kReduceScratch = 3 * (kReduceThreads / 64);
+// float4 entries holding a reduction's scratch plus the evaluated omega
+// (kReduceScratch + 1 doubles).
+constexpr int kOmegaScratchF4 = (8 * (kReduceScratch + 1) + 15) / 16;
 
 // Fixed-order sums of a scene's partials (component 0, and 1-2 when `all3`)
 // by one workgroup of kReduceThreads threads: each thread adds its strided
@@ -575,10 +578,14 @@ __global__ void __launch_bounds__(kPatchThreads)
   // rest positions; w holds the start of the vertex's run in s_sorted
   __shared__ float4 s_rest[kPatchVerts];
   // corner corrections sorted by vertex (deterministic) or per-vertex LDS
-  // atomics; between patches also the scratch of the omega evaluation
+  // atomics (the first kPatchVerts entries); the omega evaluation's scratch
+  // sits in the last entries, which the atomic accumulators never reach and
+  // which the deterministic corner writes only fill after the workgroup
+  // barrier that follows every read of the evaluated omega
   __shared__ float4 s_sorted[4 * kPatchTets];
   float4* s_acc = s_sorted;
-  double* red = reinterpret_cast<double*>(s_sorted);
+  static_assert(kPatchVerts + kOmegaScratchF4 <= 4 * kPatchTets, "omega scratch overlaps s_acc");
+  double* red = reinterpret_cast<double*>(s_sorted + 4 * kPatchTets - kOmegaScratchF4);
   const int tid = threadIdx.x;
   const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
 
@@ -885,7 +892,7 @@ __global__ void k_final_blend(const SceneDev* __restrict__ scenes,
   const WorkItem wi = map[blockIdx.x];
   const SceneDev& S = scenes[wi.scene];
   const int v = wi.first + threadIdx.x;
-  if (v >= S.n_verts || n_iter <= 0) return;
+  if (v >= S.n_verts || n_iter <= 0 || !S.fused) return;
   const int k = n_iter - 1;
   const float4 y = S.ybuf[k % 3][v];
   const float4 pr = S.ybuf[k == 0 ? 0 : (k - 1) % 3][v];  // y itself at k == 0

@@ -22,7 +22,7 @@ XPBD_ENOMEM = -3
 XPBD_ESTATE = -4
 XPBD_ENUMERIC = -5
 XPBD_MAX_SDF = 8
-XPBD_ABI_VERSION = 3
+XPBD_ABI_VERSION = 4
 
 ACCUM_ATOMIC = 0
 ACCUM_CSR = 1
@@ -45,7 +45,7 @@ EXPORTED = (
     "xpbd_set_sdfs", "xpbd_set_gravity", "xpbd_compute_lumped_mass",
     "xpbd_get_positions", "xpbd_set_positions", "xpbd_get_positions_device",
     "xpbd_set_positions_device", "xpbd_get_velocities", "xpbd_set_velocities",
-    "xpbd_get_predicted", "xpbd_get_masses", "xpbd_get_multipliers",
+    "xpbd_get_predicted", "xpbd_get_masses", "xpbd_get_degenerate_tets", "xpbd_get_multipliers",
     "xpbd_get_visual_indices", "xpbd_apply_external_forces", "xpbd_solve_sdf_collision",
     "xpbd_solve_step", "xpbd_advance", "xpbd_substep", "xpbd_frame", "xpbd_synchronize",
     "xpbd_get_stream", "xpbd_set_logging", "xpbd_log_size", "xpbd_get_log", "xpbd_clear_log",
@@ -66,6 +66,7 @@ class XpbdSdf(ctypes.Structure):
         ("points", ctypes.POINTER(ctypes.c_float)),
         ("n_points", ctypes.c_int32),
         ("pad", ctypes.c_int32),
+        ("vel", ctypes.c_double * 3),
     ]
 
 
@@ -169,6 +170,8 @@ def load() -> ctypes.CDLL:
         "xpbd_get_predicted": [vp, fp],
         "xpbd_get_masses": [vp, fp, fp],
         "xpbd_get_multipliers": [vp, fp, fp],
+        "xpbd_get_degenerate_tets": [vp, ctypes.POINTER(ctypes.c_int32),
+                                     ctypes.POINTER(ctypes.c_int32)],
         "xpbd_get_visual_indices": [vp, ctypes.POINTER(ctypes.c_uint32)],
         "xpbd_apply_external_forces": [vp, ctypes.c_double],
         "xpbd_solve_sdf_collision": [vp, ctypes.c_double],

@@ -197,6 +197,7 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         h = _abi.ctypes.c_void_p()
         _abi.check(self._lib.xpbd_create(_ref(d), _ref(h)))
         self._h = h.value
+        self._warn_degenerate()
         _abi.check(self._lib.xpbd_set_exec_path(self._h, _PATHS[self._exec_path]))
         self._pushed = self._param_key()
         self._logging = None

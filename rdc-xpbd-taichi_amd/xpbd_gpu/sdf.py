@@ -48,7 +48,9 @@ class _Scalar:
 
 
 class SdfModel:
-    """Base collider (sdf.py:6-27).  The obstacle velocity is always zero."""
+    """Base collider (sdf.py:6-27).  `vel[None]` is the obstacle's velocity
+    (zero unless set): check() counts a vertex moving with v as inside when
+    n . (vel - v) > 0 (sdf.py:20-24)."""
 
     kind = None
 
@@ -56,13 +58,22 @@ class SdfModel:
         self.vel = _Scalar(_vec((0.0, 0.0, 0.0)))
         self.fixed = _Scalar(int(bool(fixed)))
 
+    def _abi_struct(self) -> _abi.XpbdSdf:
+        s = _abi.XpbdSdf()
+        s.kind = self.kind
+        s.fixed = int(bool(self.fixed[None]))
+        v = np.asarray(self.vel[None], dtype=np.float64)
+        for k in range(3):
+            s.vel[k] = float(v[k])
+        return s
+
     def to_abi(self) -> _abi.XpbdSdf:
         raise NotImplementedError
 
     def key(self):
         """What the device copy depends on (a change is pushed before the next step)."""
         a = self.to_abi()
-        return (a.kind, a.fixed, tuple(a.p), tuple(a.dir))
+        return (a.kind, a.fixed, tuple(a.p), tuple(a.dir), tuple(a.vel))
 
 
 class PlaneSdf(SdfModel):
@@ -81,9 +92,7 @@ class PlaneSdf(SdfModel):
         self._dir64 = d / n
 
     def to_abi(self) -> _abi.XpbdSdf:
-        s = _abi.XpbdSdf()
-        s.kind = self.kind
-        s.fixed = int(bool(self.fixed[None]))
+        s = self._abi_struct()
         pt = np.asarray(self.point[None], dtype=np.float64)
         for k in range(3):
             s.p[k] = float(pt[k])
@@ -102,9 +111,7 @@ class SphereSdf(SdfModel):
         self.radius = _Scalar(float(radius))
 
     def to_abi(self) -> _abi.XpbdSdf:
-        s = _abi.XpbdSdf()
-        s.kind = self.kind
-        s.fixed = int(bool(self.fixed[None]))
+        s = self._abi_struct()
         c = np.asarray(self.center[None], dtype=np.float64)
         for k in range(3):
             s.p[k] = float(c[k])
@@ -125,9 +132,7 @@ class CubeSdf(SdfModel):
         self.half_size = _Scalar(_vec(np.asarray(size, dtype=np.float64).reshape(3) * 0.5))
 
     def to_abi(self) -> _abi.XpbdSdf:
-        s = _abi.XpbdSdf()
-        s.kind = self.kind
-        s.fixed = int(bool(self.fixed[None]))
+        s = self._abi_struct()
         c = np.asarray(self.center[None], dtype=np.float64)
         h = np.asarray(self.half_size[None], dtype=np.float64)
         for k in range(3):
@@ -172,8 +177,7 @@ class HangSdfModel(SdfModel):
         self.sphere_radius = 0.013
 
     def to_abi(self) -> _abi.XpbdSdf:
-        s = _abi.XpbdSdf()
-        s.kind = self.kind
+        s = self._abi_struct()
         s.fixed = 1
         s.p[3] = float(self.sphere_radius)
         a = self.sphere_pos._a  # held by this object for the duration of the call
@@ -182,7 +186,8 @@ class HangSdfModel(SdfModel):
         return s
 
     def key(self):
-        return (self.kind, float(self.sphere_radius), self.sphere_pos._a.tobytes())
+        return (self.kind, float(self.sphere_radius), self.sphere_pos._a.tobytes(),
+                tuple(np.asarray(self.vel[None], dtype=np.float64)))
 
 
 def as_sdf_list(sdf):

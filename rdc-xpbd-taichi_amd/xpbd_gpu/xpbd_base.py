@@ -10,6 +10,7 @@ the C ABI and exposes the device buffers as field-like objects
 from __future__ import annotations
 
 import ctypes
+import warnings
 
 import numpy as np
 
@@ -196,6 +197,25 @@ class XPBDSolverBase:
         _abi.check(self._lib.xpbd_get_masses(self._h, _abi.fptr(m), _abi.fptr(im)))
         return m, im
 
+    def degenerate_tets(self):
+        """(count, first): the tets whose volume was at most EPSILON when the
+        masses were last computed, and the first of them (-1 when none)."""
+        n = ctypes.c_int32()
+        first = ctypes.c_int32()
+        _abi.check(self._lib.xpbd_get_degenerate_tets(self._h, ctypes.byref(n),
+                                                       ctypes.byref(first)))
+        return n.value, first.value
+
+    def _warn_degenerate(self):
+        """The reference prints a warning per cell of volume <= EPSILON from its
+        mass kernel and goes on (xpbd_base.py:104-106); one warning names the
+        first such cell and their number."""
+        n, first = self.degenerate_tets()
+        if n:
+            warnings.warn(f"Warning: Cell {first} has negative or zero volume "
+                          f"({n} cell{'s' if n > 1 else ''} in total)", RuntimeWarning,
+                          stacklevel=3)
+
     def _get_multipliers(self):
         ld = np.empty(self.n_tets, dtype=np.float32)
         lh = np.empty(self.n_tets, dtype=np.float32)
@@ -207,6 +227,7 @@ class XPBDSolverBase:
         """Masses from tet volumes scaled to total_mass, then fixed-SDF pinning
         (xpbd_base.py:133-137)."""
         _abi.check(self._lib.xpbd_compute_lumped_mass(self._h, float(self.total_mass)))
+        self._warn_degenerate()
 
     def apply_external_forces(self, dt):
         """v += g dt for free vertices; pred_x = x + v dt (xpbd_base.py:139-146)."""
