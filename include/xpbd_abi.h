@@ -266,6 +266,11 @@ int xpbd_batch_synchronize(xpbd_batch* b);
 int xpbd_batch_get_stream(xpbd_batch* b, void** stream);
 int xpbd_batch_info(xpbd_batch* b, int32_t* n_scenes, int64_t* n_verts, int64_t* n_tets,
                     int32_t* n_patches);
+/* Persistent grids of the batch's patch kernels (workgroups of the projection
+ * and of the residual) and the largest number of scenes one projection
+ * workgroup's patch range touches (2 or more: ranges cross scenes). */
+int xpbd_batch_grid(xpbd_batch* b, int32_t* grid_project, int32_t* grid_residual,
+                    int32_t* max_scenes_per_range);
 
 /* Host-only description of the patch layout a mesh would get (no device
  * work): number of patches, number of (patch, vertex) slots, and the largest

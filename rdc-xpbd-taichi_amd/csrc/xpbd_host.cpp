@@ -1683,6 +1683,7 @@ struct xpbd_batch {
   DevBuf<PatchItem> pitems;
   DevBuf<int2> prange_proj, prange_res;
   int nvb = 0, n_items = 0, grid_proj = 0, grid_res = 0;
+  int max_scenes_per_range = 0;
   int64_t n_verts = 0, n_tets = 0;
   SceneDev proto{};  // flags and iteration count shared by the members
   hipGraphExec_t gexec = nullptr;
@@ -1806,6 +1807,12 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes) {
   std::vector<int2> rp, rr;
   TRY(xcd_ranges(b->n_items, b->grid_proj, nx, rp));
   TRY(grouped_ranges(items, b->grid_res, nx, rr, &b->grid_res));
+  b->max_scenes_per_range = 0;
+  for (const int2& r : rp) {
+    int k = r.y > r.x ? 1 : 0;
+    for (int i = r.x + 1; i < r.y; ++i) k += items[i].scene != items[i - 1].scene;
+    b->max_scenes_per_range = std::max(b->max_scenes_per_range, k);
+  }
   TRY(b->prange_proj.upload_reuse(rp.data(), rp.size()));
   TRY(b->prange_res.upload_reuse(rr.data(), rr.size()));
   if (b->table.cap < size_t(n_scenes)) TRY(b->table.alloc(n_scenes));
@@ -1991,6 +1998,15 @@ int xpbd_batch_synchronize(xpbd_batch* b) {
 int xpbd_batch_get_stream(xpbd_batch* b, void** stream) {
   if (!b || !stream) return set_error(XPBD_EINVAL, "NULL argument");
   *stream = static_cast<void*>(b->stream);
+  return XPBD_OK;
+}
+
+int xpbd_batch_grid(xpbd_batch* b, int32_t* grid_project, int32_t* grid_residual,
+                    int32_t* max_scenes_per_range) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  if (grid_project) *grid_project = b->grid_proj;
+  if (grid_residual) *grid_residual = b->grid_res;
+  if (max_scenes_per_range) *max_scenes_per_range = b->max_scenes_per_range;
   return XPBD_OK;
 }
 

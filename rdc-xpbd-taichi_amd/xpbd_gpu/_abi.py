@@ -53,7 +53,7 @@ EXPORTED = (
     "xpbd_set_kernel_timing", "xpbd_get_kernel_time", "xpbd_check_finite", "xpbd_last_error",
     "xpbd_patch_stats", "xpbd_get_device_bytes", "xpbd_batch_opts_init", "xpbd_batch_create",
     "xpbd_batch_destroy", "xpbd_batch_set_members", "xpbd_batch_substep", "xpbd_batch_frame",
-    "xpbd_batch_synchronize", "xpbd_batch_get_stream", "xpbd_batch_info",
+    "xpbd_batch_synchronize", "xpbd_batch_get_stream", "xpbd_batch_info", "xpbd_batch_grid",
 )
 
 
@@ -206,6 +206,8 @@ def load() -> ctypes.CDLL:
         "xpbd_batch_get_stream": [vp, pp],
         "xpbd_batch_info": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)],
+        "xpbd_batch_grid": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                            ctypes.POINTER(ctypes.c_int32)],
         "xpbd_patch_stats": [fp, ctypes.POINTER(ctypes.c_int32), i32, i32,
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],

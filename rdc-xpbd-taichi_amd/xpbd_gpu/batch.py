@@ -67,6 +67,16 @@ class SceneBatch:
         self.iters = int(s0.constraints_iter)
         self.substeps_num = int(s0.substeps_num)
 
+    def grid(self) -> dict:
+        """Persistent grids of the patch kernels and the most scenes one
+        projection workgroup's patch range touches (xpbd_batch_grid)."""
+        gp = ctypes.c_int32()
+        gr = ctypes.c_int32()
+        ms = ctypes.c_int32()
+        _abi.check(self._lib.xpbd_batch_grid(self._h, ctypes.byref(gp), ctypes.byref(gr),
+                                             ctypes.byref(ms)))
+        return {"project": gp.value, "residual": gr.value, "max_scenes_per_range": ms.value}
+
     def set_members(self, solvers):
         """Re-targets the batch at another list of scenes (xpbd_batch_set_members):
         the stream and large-enough device buffers are kept; queued work is

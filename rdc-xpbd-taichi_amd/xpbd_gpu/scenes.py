@@ -164,6 +164,8 @@ def make_solver(scene: Scene, **opts):
             h = HangSdfModel(np.asarray(s["points"], dtype=np.float32))
             h.sphere_radius = s.get("radius", 0.013)
             sd.append(h)
+        if "vel" in s:
+            sd[-1].vel[None] = s["vel"]
     p = dict(scene.params)
     solver = XPBDNeoHookeanSolver((scene.mesh.verts, scene.mesh.tets), sd, **p, **opts)
     solver.gravity[None] = scene.gravity

@@ -18,6 +18,60 @@ def make_oracle(scene: Scene, precision=32, threads=None, **over):
     return o
 
 
+def rounding_variants(scene: Scene, frames=None, substeps=None, seeds=(1, 2), threads=None,
+                      setup=None, **over):
+    """The f32 oracle on the same scene and schedule with its rounding changed,
+    once per seed: the first run takes every scatter and sum in a seeded
+    random order (oracle.VAR_ORDER; the reference's float atomics fix no
+    order), the others also move 1/x and sqrt by a seeded -1/0/+1 ulp
+    (VAR_ULP, the native GPU instructions' accuracy), which perturbs even a
+    mesh too small for order to matter.  The spread of these runs around the
+    default one is the f32 noise of the case itself.  `setup(o)` prepares each
+    oracle like the caller's own.  Returns a list of (positions, log)."""
+    from oracle import VAR_ORDER, VAR_ULP
+    out = []
+    for i, seed in enumerate(seeds):
+        var = VAR_ORDER if i == 0 else VAR_ORDER | VAR_ULP
+        o = make_oracle(scene, threads=threads, variant=var, variant_seed=seed, **over)
+        if setup is not None:
+            setup(o)
+        o.set_log(True)
+        if substeps is not None:
+            sub_dt = scene.params["dt"] / scene.params["substeps_num"]
+            for _ in range(substeps):
+                o.substep(sub_dt)
+        else:
+            o.frame(frames if frames is not None else scene.frames)
+        out.append((o.x, o.log()))
+    return out
+
+
+def position_spread(x0, variants) -> float:
+    """Largest distance of a rounding-variant run from the default run."""
+    return max((float(np.abs(x - x0).max()) for x, _ in variants), default=0.0)
+
+
+def rhat_floor(log0, variants, agree=0.005):
+    """The f32 floor of a residual curve: the r_hat level above which every
+    rounding-variant run agrees with the default run within `agree` (relative);
+    iterations below it are round-off.  0.0 when they agree everywhere, None
+    when they disagree even at the largest r_hat."""
+    r0 = log0[:, 3]
+    spread = np.zeros_like(r0)
+    for _, lg in variants:
+        with np.errstate(divide="ignore", invalid="ignore"):
+            rel = np.abs(lg[:, 3] - r0) / r0
+        spread = np.maximum(spread, np.where(r0 > 0, rel, np.inf))
+    order = np.argsort(-r0, kind="stable")
+    worst = np.maximum.accumulate(spread[order])
+    bad = np.nonzero(worst > agree)[0]
+    if bad.size == 0:
+        return 0.0
+    if bad[0] == 0:
+        return None
+    return float(r0[order[bad[0]]])
+
+
 def fan_mesh(n_tets=600, pool=30, seed=0):
     """n_tets tets that all share vertex 0 (at the origin), their other three
     vertices drawn from `pool` points on the unit sphere; each tet is

@@ -14,7 +14,8 @@ def _print_case(name, **kw):
 
 
 @pytest.mark.parametrize("opts", [dict(), dict(graph=False), dict(graph_substeps=5),
-                                  dict(xcd_split=False, vertex_block=128, grid_pct=50)])
+                                  dict(xcd_split=False, vertex_block=128, grid_pct=50),
+                                  dict(grid_pct=1), dict(grid_pct=1, graph=False)])
 def test_batch_members_match_alone_bitwise(opts):
     _print_case("batch_vs_alone", **opts)
     scenes = batched_scenes(n_scenes=6, seed=3, n_range=(3, 9), iters=20)
@@ -24,6 +25,8 @@ def test_batch_members_match_alone_bitwise(opts):
         s.solve()
         s.solve()
     with SceneBatch(mixed, **opts) as b:
+        if opts.get("grid_pct") == 1:  # workgroups walk patches of several scenes
+            assert b.grid()["max_scenes_per_range"] >= 2, b.grid()
         b.frame(2)
         b.synchronize()
     for a, m in zip(alone, mixed):
@@ -67,17 +70,17 @@ def test_batch_substeps_and_mixed_sizes():
         assert np.array_equal(a.mesh.verts.x.to_numpy(), m.mesh.verts.x.to_numpy())
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_set_members_matches_fresh_batches(graph):
+@pytest.mark.parametrize("graph,grid_pct", [(False, 100), (True, 100), (True, 1)])
+def test_set_members_matches_fresh_batches(graph, grid_pct):
     """A batch re-targeted at other members (growing, shrinking and reordering
     its list) steps each scene exactly as plain solves do."""
-    _print_case("set_members", graph=graph)
+    _print_case("set_members", graph=graph, grid_pct=grid_pct)
     scenes = batched_scenes(n_scenes=5, seed=11, n_range=(3, 10), iters=15)
     alone = [make_solver(s) for s in scenes]
     mixed = [make_solver(s) for s in scenes]
     plan = [[0, 1], [1, 2, 3, 4], [4], [3, 0, 2]]  # members per substep round
     done = [0] * len(scenes)
-    with SceneBatch([mixed[0]], graph=graph) as b:
+    with SceneBatch([mixed[0]], graph=graph, grid_pct=grid_pct) as b:
         for members in plan:
             b.set_members([mixed[k] for k in members])
             assert b.n_scenes == len(members)
