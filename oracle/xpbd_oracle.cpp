@@ -34,8 +34,9 @@
 // Variants (Params.variant, bit flags; 0 is the restatement itself):
 //   kVarOrder  every scatter and every sum is taken in a seeded random order
 //              (the reference's float atomics fix no order)
-//   kVarUlp    1/x and sqrt are perturbed by a seeded -1, 0 or +1 ulp where a
-//              GPU would use its native 1-ulp instructions (rest-shape
+//   kVarUlp    1/x and sqrt are rounded faithfully (exact when representable,
+//              otherwise either neighbouring float, by a seeded choice) where
+//              a GPU would use its native 1-ulp instructions (rest-shape
 //              inverse, Jacobi average, residual), and x / c is formed as
 //              x * (1/c) there
 //   kVarFold   the compliance is formed as (6 / (lame dt^2)) * (1 / det Dm)
@@ -79,11 +80,30 @@ inline int ulp_step(uint64_t seed, uint64_t a, uint64_t b) {
   return int(h % 3) - 1;
 }
 
-// x moved by `step` ulps (f32 only; the f64 run is never perturbed).
+// A faithfully rounded f32 result for the exact value `exact` (computed in
+// double from f32 operands): the value itself when it is representable,
+// otherwise one of the two floats around it -- the nearest for `step` 1 or
+// 2, the other one for `step` 3 (ulp_step() + 2 picks each with probability
+// 1/3).  This is the accuracy of 1-ulp instructions: exact results stay
+// exact.  (Rounding the double to float is the correctly rounded f32 result:
+// double carries more than 2 x 24 + 2 bits.)
+inline float faithful_f32(double exact, int step) {
+  const float f = float(exact);
+  if (double(f) == exact || step < 3) return f;
+  return std::nextafter(f, double(f) < exact ? INFINITY : -INFINITY);
+}
+
+// 1/x and sqrt(x) as the kVarUlp variant forms them (f32 only; the f64 run
+// and step 0 give the correctly rounded result, by the ordinary operation).
 template <typename R>
-inline R nudge(R x, int step) {
-  if (sizeof(R) != sizeof(float) || step == 0) return x;
-  return R(std::nextafter(float(x), step > 0 ? INFINITY : -INFINITY));
+inline R recip(R x, int step) {
+  if (sizeof(R) != sizeof(float) || step == 0) return R(1) / x;
+  return R(faithful_f32(1.0 / double(x), step));
+}
+template <typename R>
+inline R root(R x, int step) {
+  if (sizeof(R) != sizeof(float) || step == 0) return std::sqrt(x);
+  return R(faithful_f32(std::sqrt(double(x)), step));
 }
 
 template <typename R>
@@ -114,12 +134,13 @@ inline R det3(const M3<R>& a) {
          m[2][0] * (m[0][1] * m[1][2] - m[1][1] * m[0][2]);
 }
 
-// 3x3 inverse as adjugate / determinant (Matrix.inverse for n == 3); `step`
-// moves the reciprocal of the determinant by that many ulps (kVarUlp).
+// 3x3 inverse as adjugate / determinant (Matrix.inverse for n == 3); a
+// non-zero `step` forms the reciprocal of the determinant faithfully
+// rounded (kVarUlp).
 template <typename R>
 inline M3<R> inv3(const M3<R>& a, int step = 0) {
   const auto& m = a.m;
-  const R inv_det = nudge(R(1) / det3(a), step);
+  const R inv_det = recip(det3(a), step);
   M3<R> r;
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j) {
@@ -348,7 +369,7 @@ inline int split_projection(const R* const p[4], const R w[4], const M3<R>& idm,
 // compute_xpbd_residual for one tet (neohookean_solver.py:298-329):
 // r = sqrt((C_D + alpha~_D lambda_D)^2 + (C_H + alpha~_H lambda_H)^2); C_D
 // and C_H (the monitor kernels' quantities, :256-296) go to *C_D / *C_H.
-// `step` moves the square root by that many ulps (kVarUlp).
+// A non-zero `step` forms the square root faithfully rounded (kVarUlp).
 template <typename R>
 inline R residual_value(const R* const x[4], const M3<R>& idm, R aD, R aH, R lam_d, R lam_h,
                         R* C_D, R* C_H, int step = 0) {
@@ -357,7 +378,7 @@ inline R residual_value(const R* const x[4], const M3<R>& idm, R aD, R aH, R lam
   *C_H = det3(F) - R(1);
   const R rD = *C_D + aD * lam_d;
   const R rH = *C_H + aH * lam_h;
-  return nudge(std::sqrt(rD * rD + rH * rH), step);
+  return root(rD * rD + rH * rH, step);
 }
 
 // _chebyshev_update (neohookean_solver.py:131-137) on n scalars: with x the
@@ -645,9 +666,9 @@ class Solver {
         dm.m[r][1] = p[2][r] - p[0][r];
         dm.m[r][2] = p[3][r] - p[0][r];
       }
-      const int step = (variant_ & kVarUlp) ? ulp_step(seed_, uint64_t(t), 0x1d) : 0;
+      const int step = (variant_ & kVarUlp) ? ulp_step(seed_, uint64_t(t), 0x1d) + 2 : 0;
       inv_dm_[t] = inv3(dm, step);
-      inv_det_[t] = nudge(R(1) / det3(dm), step);
+      inv_det_[t] = recip(det3(dm), step);
     }
   }
 
@@ -821,7 +842,7 @@ class Solver {
       cons_[v] = cnt;
       if (cnt > 0) {
         if (variant_ & kVarUlp) {
-          const R ic = nudge(R(1) / R(cnt), ulp_step(seed_, uint64_t(v), 2 * iter_count_ + 1));
+          const R ic = recip(R(cnt), ulp_step(seed_, uint64_t(v), 2 * iter_count_ + 1) + 2);
           for (int k = 0; k < 3; ++k) pred_[3 * v + k] += d[k] * ic;
         } else {
           for (int k = 0; k < 3; ++k) pred_[3 * v + k] += d[k] / R(cnt);
@@ -844,7 +865,7 @@ class Solver {
       R aD, aH, C_D, C_H;
       compliance(lame1_, lame2_, V0_[t], inv_det_[t], dt, variant_, &aD, &aH);
       const int step =
-          (variant_ & kVarUlp) ? ulp_step(seed_, uint64_t(t), 2 * iter_count_ + 2) : 0;
+          (variant_ & kVarUlp) ? ulp_step(seed_, uint64_t(t), 2 * iter_count_ + 2) + 2 : 0;
       per_tet_[t] = residual_value(x, inv_dm_[t], aD, aH, lam_d_[t], lam_h_[t], &C_D, &C_H, step);
       per_tet2_[t] = C_H;
       per_tet3_[t] = C_D;

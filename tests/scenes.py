@@ -18,21 +18,32 @@ def make_oracle(scene: Scene, precision=32, threads=None, **over):
     return o
 
 
-def rounding_variants(scene: Scene, frames=None, substeps=None, seeds=(1, 2), threads=None,
-                      setup=None, **over):
-    """The f32 oracle on the same scene and schedule with its rounding changed,
-    once per seed: the first run takes every scatter and sum in a seeded
-    random order (oracle.VAR_ORDER; the reference's float atomics fix no
-    order), the others also move 1/x and sqrt by a seeded -1/0/+1 ulp
-    (VAR_ULP, the native GPU instructions' accuracy), which perturbs even a
-    mesh too small for order to matter.  The spread of these runs around the
-    default one is the f32 noise of the case itself.  `setup(o)` prepares each
-    oracle like the caller's own.  Returns a list of (positions, log)."""
-    from oracle import VAR_ORDER, VAR_ULP
+# The rounding variants of the f32 oracle: (library flavour, variant flags, seed).
+#  1. every scatter and sum in a seeded random order (oracle.VAR_ORDER): the
+#     reference's float atomics fix no order;
+#  2. the same, with 1/x and sqrt also moved by a seeded -1/0/+1 ulp
+#     (VAR_ULP), which perturbs even a mesh too small for order to matter;
+#  3. the same, in the oracle built with floating-point contraction (fused
+#     multiply-add) and with the compliance reassociated as (6 / (lame dt^2))
+#     / det Dm (VAR_FOLD): the reference's driver initialises Taichi with its
+#     defaults (ti.init(arch=...), run_neohookean_usd.py:81), and Taichi's
+#     default fast-math mode permits contraction, reassociation and
+#     approximate 1/x and sqrt.
+ROUNDING_VARIANTS = (("plain", 1, 1), ("plain", 3, 2), ("fma", 7, 3))
+
+
+def rounding_variants(scene: Scene, frames=None, substeps=None, threads=None, setup=None,
+                      kinds=ROUNDING_VARIANTS, repeats=1, **over):
+    """The f32 oracle on the same scene and schedule once per rounding variant
+    (ROUNDING_VARIANTS), each kind `repeats` times with other seeds; their
+    spread around the default run is the f32 noise of the case itself (more
+    repeats sample it better where it is wide).  `setup(o)` prepares each
+    oracle like the caller's own.  Returns a list of (positions, log, flavour)."""
     out = []
-    for i, seed in enumerate(seeds):
-        var = VAR_ORDER if i == 0 else VAR_ORDER | VAR_ULP
-        o = make_oracle(scene, threads=threads, variant=var, variant_seed=seed, **over)
+    kinds = [(f, v, seed + 100 * r) for r in range(repeats) for f, v, seed in kinds]
+    for flavor, var, seed in kinds:
+        o = make_oracle(scene, threads=threads, variant=var, variant_seed=seed, flavor=flavor,
+                        **over)
         if setup is not None:
             setup(o)
         o.set_log(True)
@@ -42,26 +53,30 @@ def rounding_variants(scene: Scene, frames=None, substeps=None, seeds=(1, 2), th
                 o.substep(sub_dt)
         else:
             o.frame(frames if frames is not None else scene.frames)
-        out.append((o.x, o.log()))
+        out.append((o.x, o.log(), flavor))
     return out
 
 
 def position_spread(x0, variants) -> float:
-    """Largest distance of a rounding-variant run from the default run."""
-    return max((float(np.abs(x - x0).max()) for x, _ in variants), default=0.0)
+    """The f32 noise of a case's final positions: the largest distance between
+    two of the runs that differ only in rounding (the default run x0 and the
+    rounding variants)."""
+    xs = [np.asarray(x0, np.float64)] + [np.asarray(v[0], np.float64) for v in variants]
+    return max((float(np.abs(a - b).max()) for i, a in enumerate(xs) for b in xs[i + 1:]),
+               default=0.0)
 
 
 def rhat_floor(log0, variants, agree=0.005):
     """The f32 floor of a residual curve: the r_hat level above which every
-    rounding-variant run agrees with the default run within `agree` (relative);
+    rounding variant agrees with the default run within `agree` (relative);
     iterations below it are round-off.  0.0 when they agree everywhere, None
     when they disagree even at the largest r_hat."""
     r0 = log0[:, 3]
     spread = np.zeros_like(r0)
-    for _, lg in variants:
+    for v in variants:
         with np.errstate(divide="ignore", invalid="ignore"):
-            rel = np.abs(lg[:, 3] - r0) / r0
-        spread = np.maximum(spread, np.where(r0 > 0, rel, np.inf))
+            rel = np.where(r0 > 0, np.abs(v[1][:, 3] - r0) / r0, np.inf)
+        spread = np.maximum(spread, rel)
     order = np.argsort(-r0, kind="stable")
     worst = np.maximum.accumulate(spread[order])
     bad = np.nonzero(worst > agree)[0]

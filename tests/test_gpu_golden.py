@@ -2,12 +2,15 @@
 
 Each fixture is run through the C ABI from its stored inputs (and starting
 state) and compared with the fixture's f32 oracle positions and residual log.
-The f32 oracle is also run twice more from the same inputs with its rounding
-changed (scatter and sum order permuted, then also 1/x and sqrt moved by up to
-one ulp; seeded), which measures the case's own f32 noise.
+The f32 oracle is also run three more times from the same inputs with its
+rounding changed (scenes.ROUNDING_VARIANTS: scatter and sum order permuted,
+then also 1/x and sqrt moved by up to one ulp, then also fused multiply-adds
+and the compliance reassociated; seeded), which measures the case's own f32
+noise.
 Gates, as in test_gpu_parity:
   positions  max |x_gpu - x_f32| <= max(1e-4 x bbox diagonal, 2 x the
-             rounding variants' largest distance from the fixture).
+             largest distance between two of the fixture's f32 run and its
+             rounding variants).
   residual   r_hat within 1 % of the fixture's above the case's f32 floor (the
              level above which the variants agree within 0.5 %).  Where that
              floor leaves less than half of the iterations gated (the

@@ -17,7 +17,7 @@ import pytest
 
 from scenes import (batched_scenes, resting_cube, dropped_cube, jittered_box, twisted_grid,
                     make_oracle)
-from scenes import position_spread
+from scenes import ROUNDING_VARIANTS, position_spread
 from test_gpu_parity import POS_TOL, check_residual_curve, make_solver
 from xpbd_gpu.batch import SceneBatch
 from xpbd_gpu.serve import Request, Server, served_scenes
@@ -59,22 +59,19 @@ def histogram(v):
     return dict(zip(labels, h.tolist()))
 
 
-def run_case(name, scene, frames=None, substeps=None, curve=True, seeds=(1, 2),
-             per_frame=False):
-    """The device, the f32 oracle, the f64 oracle and rounding variants of the
-    f32 oracle (one per seed: order permuted, then also 1-ulp reciprocals and
-    square roots) on the same scene and schedule.  The position gate is
+def run_case(name, scene, frames=None, substeps=None, curve=True, per_frame=False):
+    """The device, the f32 oracle, the f64 oracle and the rounding variants of
+    the f32 oracle (scenes.ROUNDING_VARIANTS) on the same scene and schedule.  The position gate is
     max(1e-4 x diag, 2 x the variants' largest distance from the f32 oracle),
     as in test_gpu_parity; the residual curve is gated above the f32 floor.
     With per_frame, each frame's distances are printed and the entry also
     holds them."""
-    from oracle import VAR_ORDER, VAR_ULP
     print(f"[case] report {name}", flush=True)
     solver = make_solver(scene, log_residuals=True)
     o32 = make_oracle(scene, threads=THREADS)
     o64 = make_oracle(scene, precision=64, threads=THREADS)
-    ovs = [make_oracle(scene, threads=THREADS, variant=VAR_ORDER | (VAR_ULP if i else 0),
-                       variant_seed=k) for i, k in enumerate(seeds)]
+    ovs = [make_oracle(scene, threads=THREADS, variant=var, variant_seed=seed, flavor=flavor)
+           for flavor, var, seed in ROUNDING_VARIANTS]
     for o in [o32, *ovs]:
         o.set_log(True)
     diag = scene.diag
@@ -103,7 +100,7 @@ def run_case(name, scene, frames=None, substeps=None, curve=True, seeds=(1, 2),
                 frames_log.append(row)
                 print(f"  frame {f + 1}: " + ", ".join(f"{k} {v:.2e}" for k, v in row.items()
                                                      if k != "frame"), flush=True)
-    o32.variants = [(o.x, o.log()) for o in ovs]
+    o32.variants = [(o.x, o.log(), k[0]) for o, k in zip(ovs, ROUNDING_VARIANTS)]
     x = solver.mesh.verts.x.to_numpy().astype(np.float64)
     e_gpu = float(np.abs(x - o64.x).max() / diag)
     e_f32 = float(np.abs(o32.x - o64.x).max() / diag)
@@ -192,7 +189,7 @@ def test_report_jittered_box_frame(jittered_box_run, frame):
     assert np.all(np.isfinite(x))
     assert row["gpu_vs_f32_oracle_x_diag"] <= max(POS_TOL, 2.0 * row["f32_order_spread_x_diag"])
     if frame == scene.frames:
-        o32.variants = [(ov.x, ov.log())]
+        o32.variants = [(ov.x, ov.log(), "plain")]
         worst = check_residual_curve(solver, o32)
         _write("jittered_box_ten_frames",
                {"tets": int(scene.mesh.n_tets), "verts": int(scene.mesh.n_verts),
