@@ -258,6 +258,22 @@ int xpbd_batch_destroy(xpbd_batch* b);
  * batch's queued work.  For serving loops whose membership changes at
  * substep boundaries. */
 int xpbd_batch_set_members(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_scenes);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops).  Re-targets b like
+ * xpbd_batch_set_members, but instead of buffers the batch allocates, its
+ * work lists and scene table go to a fixed array of device memory, one per
+ * GPU, that the library holds until the process ends; the call makes no
+ * device allocation or release.  Limits: at most XPBD_NOALLOC_MAX_SCENES
+ * members and XPBD_NOALLOC_LIST_BYTES of work lists (32 B per patch of 512
+ * tets, 8 B per vertex block and per persistent workgroup); larger member
+ * lists are refused with XPBD_EINVAL and leave the batch as it was.  The
+ * batch must not replay graphs (opts.graph == 0: capturing a new one would
+ * allocate).  The array holds one batch's lists per GPU: a later call for
+ * another batch on the same GPU takes it over, and the earlier batch then
+ * reports XPBD_ESTATE until re-targeted; calls that overlap on one GPU race. */
+#define XPBD_NOALLOC_MAX_SCENES 64
+#define XPBD_NOALLOC_LIST_BYTES (1 << 20)
+int xpbd_batch_set_members_noalloc(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_scenes);
 /* one substep (sub_dt = dt / substeps_num of each scene) of every member */
 int xpbd_batch_substep(xpbd_batch* b);
 /* n_frames frames of every member (solve(), neohookean_solver.py:562-587) */

@@ -18,7 +18,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -1688,6 +1690,14 @@ struct xpbd_batch {
   SceneDev proto{};  // flags and iteration count shared by the members
   hipGraphExec_t gexec = nullptr;
   int graph_substeps = 0;
+  // where the scene table and work lists live: the batch's own buffers, or
+  // (noalloc) the per-GPU array of xpbd_batch_set_members_noalloc
+  bool noalloc = false;
+  SceneDev* table_d = nullptr;
+  WorkItem* vmap_d = nullptr;
+  PatchItem* items_d = nullptr;
+  int2* rp_d = nullptr;
+  int2* rr_d = nullptr;
 
   ~xpbd_batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1696,6 +1706,17 @@ struct xpbd_batch {
 };
 
 namespace {
+
+// The batch whose lists each GPU's no-allocation array holds.
+constexpr int kMaxNoallocDevices = 64;
+std::mutex g_noalloc_mutex;
+xpbd_batch* g_noalloc_owner[kMaxNoallocDevices] = {};
+
+void noalloc_release(xpbd_batch* b) {
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (b->device >= 0 && b->device < kMaxNoallocDevices && g_noalloc_owner[b->device] == b)
+    g_noalloc_owner[b->device] = nullptr;
+}
 
 // Members of one batch share what the launch sequence depends on: the
 // iteration count, time step, substep count and mode flags (checked on the
@@ -1735,7 +1756,7 @@ int batch_sync_table(xpbd_batch* b) {
   }
   if (!changed) return XPBD_OK;
   HIP_TRY(hipStreamSynchronize(b->stream));
-  HIP_TRY(hipMemcpy(b->table.p, b->table_host.data(), sizeof(SceneDev) * b->table_host.size(),
+  HIP_TRY(hipMemcpy(b->table_d, b->table_host.data(), sizeof(SceneDev) * b->table_host.size(),
                     hipMemcpyHostToDevice));
   const SceneDev& p = b->table_host[0];
   if (p.iters != b->proto.iters || p.fused != b->proto.fused || p.cheb != b->proto.cheb ||
@@ -1772,51 +1793,98 @@ int batch_check_members(xpbd_solver* const* scenes, int n, int device) {
 
 // The batch's concatenated work lists (vertex blocks and patch items in member
 // order), its persistent grids and ranges, and its scene table, for the given
-// members; device buffers large enough from an earlier membership are reused.
-int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes) {
+// members.  They go to the batch's own device buffers (reused when large
+// enough from an earlier membership) or, with `noalloc`, to the per-GPU
+// no-allocation array; a membership that does not fit there is refused
+// before the batch changes.
+int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes, bool noalloc) {
   const xpbd_batch_opts& o = b->opts;
-  b->scenes.assign(scenes, scenes + n_scenes);
-  b->n_verts = b->n_tets = 0;
   std::vector<WorkItem> vm;
   std::vector<PatchItem> items;
+  int64_t n_verts = 0, n_tets = 0;
   for (int i = 0; i < n_scenes; ++i) {
-    xpbd_solver* s = b->scenes[i];
+    xpbd_solver* s = scenes[i];
     for (int v = 0; v < s->nv; v += o.vertex_block) vm.push_back({i, v});
     for (PatchItem it : s->pitems_host) {
       it.scene = i;
       items.push_back(it);
     }
-    b->n_verts += s->nv;
-    b->n_tets += s->nt;
+    n_verts += s->nv;
+    n_tets += s->nt;
   }
   const int nx = o.xcd_split ? xcd_count() : 1;
   std::vector<WorkItem> vmo(vm.size());
   const std::vector<int> order = xcd_order(int(vm.size()), nx);
   for (size_t k = 0; k < vm.size(); ++k) vmo[k] = vm[order[k]];
-  b->nvb = int(vmo.size());
-  b->n_items = int(items.size());
-  TRY(b->vmap.upload_reuse(vmo.data(), vmo.size()));
-  TRY(b->pitems.upload_reuse(items.data(), items.size()));
+  const int n_items = int(items.size());
   int occ_p = 0, occ_r = 0, n_cu = 0;
   HIP_TRY(xpbd::patch_kernel_occupancy(&occ_p, &occ_r));
   HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
   const long long full_p = (long long)n_cu * std::max(1, occ_p);
   const long long full_r = (long long)n_cu * std::max(1, occ_r);
-  b->grid_proj = int(std::max(1LL, std::min<long long>(b->n_items, full_p * o.grid_pct / 100)));
-  b->grid_res = int(std::max(1LL, std::min<long long>(b->n_items, full_r * o.grid_pct / 100)));
+  const int grid_proj = int(std::max(1LL, std::min<long long>(n_items, full_p * o.grid_pct / 100)));
+  int grid_res = int(std::max(1LL, std::min<long long>(n_items, full_r * o.grid_pct / 100)));
   std::vector<int2> rp, rr;
-  TRY(xcd_ranges(b->n_items, b->grid_proj, nx, rp));
-  TRY(grouped_ranges(items, b->grid_res, nx, rr, &b->grid_res));
+  TRY(xcd_ranges(n_items, grid_proj, nx, rp));
+  TRY(grouped_ranges(items, grid_res, nx, rr, &grid_res));
+  if (noalloc) {
+    auto a256 = [](size_t x) { return (x + 255) / 256 * 256; };
+    const size_t bytes = a256(sizeof(WorkItem) * vmo.size()) + a256(sizeof(PatchItem) * items.size()) +
+                         a256(sizeof(int2) * rp.size()) + a256(sizeof(int2) * rr.size());
+    if (n_scenes > XPBD_NOALLOC_MAX_SCENES)
+      return set_error(XPBD_EINVAL, "%d members: a batch re-targeted without allocation holds "
+                       "at most %d", n_scenes, XPBD_NOALLOC_MAX_SCENES);
+    if (bytes > size_t(XPBD_NOALLOC_LIST_BYTES))
+      return set_error(XPBD_EINVAL, "the members' work lists need %zu bytes; a batch re-targeted "
+                       "without allocation holds at most %d", bytes, XPBD_NOALLOC_LIST_BYTES);
+    void *table = nullptr, *lists = nullptr;
+    HIP_TRY(xpbd::noalloc_scratch(b->device, &table, &lists));
+    unsigned char* at = static_cast<unsigned char*>(lists);
+    auto place = [&](auto*& dst, const auto& v) -> int {
+      using T = typename std::decay_t<decltype(v)>::value_type;
+      dst = reinterpret_cast<std::remove_reference_t<decltype(dst)>>(at);
+      if (!v.empty()) HIP_TRY(hipMemcpy(at, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+      at += a256(sizeof(T) * v.size());
+      return XPBD_OK;
+    };
+    {
+      std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+      g_noalloc_owner[b->device] = b;  // a batch that held the array loses its lists
+    }
+    TRY(place(b->vmap_d, vmo));
+    TRY(place(b->items_d, items));
+    TRY(place(b->rp_d, rp));
+    TRY(place(b->rr_d, rr));
+    b->table_d = static_cast<SceneDev*>(table);
+    b->noalloc = true;
+  } else {
+    noalloc_release(b);
+    TRY(b->vmap.upload_reuse(vmo.data(), vmo.size()));
+    TRY(b->pitems.upload_reuse(items.data(), items.size()));
+    TRY(b->prange_proj.upload_reuse(rp.data(), rp.size()));
+    TRY(b->prange_res.upload_reuse(rr.data(), rr.size()));
+    if (b->table.cap < size_t(n_scenes)) TRY(b->table.alloc(n_scenes));
+    b->table.n = size_t(n_scenes);
+    b->vmap_d = b->vmap.p;
+    b->items_d = b->pitems.p;
+    b->rp_d = b->prange_proj.p;
+    b->rr_d = b->prange_res.p;
+    b->table_d = b->table.p;
+    b->noalloc = false;
+  }
+  b->scenes.assign(scenes, scenes + n_scenes);
+  b->n_verts = n_verts;
+  b->n_tets = n_tets;
+  b->nvb = int(vmo.size());
+  b->n_items = n_items;
+  b->grid_proj = grid_proj;
+  b->grid_res = grid_res;
   b->max_scenes_per_range = 0;
   for (const int2& r : rp) {
     int k = r.y > r.x ? 1 : 0;
     for (int i = r.x + 1; i < r.y; ++i) k += items[i].scene != items[i - 1].scene;
     b->max_scenes_per_range = std::max(b->max_scenes_per_range, k);
   }
-  TRY(b->prange_proj.upload_reuse(rp.data(), rp.size()));
-  TRY(b->prange_res.upload_reuse(rr.data(), rr.size()));
-  if (b->table.cap < size_t(n_scenes)) TRY(b->table.alloc(n_scenes));
-  b->table.n = size_t(n_scenes);
   b->table_host.clear();  // forces the upload
   b->proto = b->scenes[0]->host_scene;
   TRY(batch_sync_table(b));
@@ -1828,24 +1896,23 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes) {
 // enqueue_iterations, over the concatenated work list).
 int batch_enqueue_substep(xpbd_batch* b) {
   const SceneDev& P = b->proto;
-  const SceneDev* T = b->table.p;
+  const SceneDev* T = b->table_d;
   const int bs = b->opts.vertex_block;
   const int ns = int(b->scenes.size());
-  HIP_TRY(xpbd::launch_predict(T, b->vmap.p, b->nvb, bs, 1, 1, P.cheb, b->stream));
+  HIP_TRY(xpbd::launch_predict(T, b->vmap_d, b->nvb, bs, 1, 1, P.cheb, b->stream));
   for (int it = 0; it < P.iters; ++it) {
-    HIP_TRY(xpbd::launch_patch_project(T, b->pitems.p, b->prange_proj.p, b->grid_proj, it,
-                                       ns > 1, b->stream));
-    HIP_TRY(xpbd::launch_patch_average(T, b->vmap.p, b->nvb, bs, it, b->stream));
+    HIP_TRY(xpbd::launch_patch_project(T, b->items_d, b->rp_d, b->grid_proj, it, ns > 1,
+                                       b->stream));
+    HIP_TRY(xpbd::launch_patch_average(T, b->vmap_d, b->nvb, bs, it, b->stream));
     if (P.cheb || P.monitor) {
-      HIP_TRY(xpbd::launch_patch_residual(T, b->pitems.p, b->prange_res.p, b->grid_res,
-                                          b->stream));
+      HIP_TRY(xpbd::launch_patch_residual(T, b->items_d, b->rr_d, b->grid_res, b->stream));
       if (!P.fused || it == P.iters - 1) HIP_TRY(xpbd::launch_reduce(T, ns, it, b->stream));
     }
-    if (P.cheb && !P.fused) HIP_TRY(xpbd::launch_blend(T, b->vmap.p, b->nvb, bs, b->stream));
+    if (P.cheb && !P.fused) HIP_TRY(xpbd::launch_blend(T, b->vmap_d, b->nvb, bs, b->stream));
   }
   if (P.fused && P.iters > 0)
-    HIP_TRY(xpbd::launch_final_blend(T, b->vmap.p, b->nvb, bs, P.iters, b->stream));
-  HIP_TRY(xpbd::launch_advance(T, b->vmap.p, b->nvb, bs, b->stream));
+    HIP_TRY(xpbd::launch_final_blend(T, b->vmap_d, b->nvb, bs, P.iters, b->stream));
+  HIP_TRY(xpbd::launch_advance(T, b->vmap_d, b->nvb, bs, b->stream));
   return XPBD_OK;
 }
 
@@ -1868,6 +1935,12 @@ bool batch_at_frame_start(const xpbd_batch* b) {
 }
 
 int batch_run(xpbd_batch* b, int n_substeps) {
+  if (b->noalloc) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (g_noalloc_owner[b->device] != b)
+      return set_error(XPBD_ESTATE, "another batch's lists replaced this batch's in the "
+                                    "no-allocation array of device %d: re-target it", b->device);
+  }
   TRY(batch_sync_table(b));
   if (!b->opts.graph) {
     for (int k = 0; k < n_substeps; ++k) {
@@ -1942,7 +2015,7 @@ int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_b
   HIP_TRY(hipSetDevice(b->device));
   TRY(batch_check_members(scenes, n_scenes, b->device));
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  TRY(batch_build(b.get(), scenes, n_scenes));
+  TRY(batch_build(b.get(), scenes, n_scenes, false));
   *out = b.release();
   return XPBD_OK;
 }
@@ -1958,13 +2031,28 @@ int xpbd_batch_set_members(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_
     b->gexec = nullptr;
     b->graph_substeps = 0;
   }
-  return batch_build(b, scenes, n_scenes);
+  return batch_build(b, scenes, n_scenes, false);
+}
+
+int xpbd_batch_set_members_noalloc(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_scenes) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  if (!scenes || n_scenes <= 0) return set_error(XPBD_EINVAL, "batch needs scenes");
+  if (b->opts.graph)
+    return set_error(XPBD_EINVAL, "a batch re-targeted without allocation must not replay "
+                                  "graphs (opts.graph = 0)");
+  if (b->device < 0 || b->device >= kMaxNoallocDevices)
+    return set_error(XPBD_EINVAL, "device %d out of range", b->device);
+  HIP_TRY(hipSetDevice(b->device));
+  TRY(batch_check_members(scenes, n_scenes, b->device));
+  HIP_TRY(hipStreamSynchronize(b->stream));  // the lists below are in use until then
+  return batch_build(b, scenes, n_scenes, true);
 }
 
 int xpbd_batch_destroy(xpbd_batch* b) {
   if (!b) return XPBD_OK;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  noalloc_release(b);
   delete b;
   return XPBD_OK;
 }

@@ -39,5 +39,9 @@ hipError_t launch_scatter_xyz(float4* dst, const float* src, const int* old_to_n
 hipError_t launch_gather_xyz(float* dst, const float4* src, const int* old_to_new, int n,
                              hipStream_t st);
 hipError_t launch_check_finite(const float4* pos, int n, int* flag, hipStream_t st);
+// Device addresses, on `device` (which must be current), of the no-allocation
+// scene table (XPBD_NOALLOC_MAX_SCENES entries) and work-list area
+// (XPBD_NOALLOC_LIST_BYTES).
+hipError_t noalloc_scratch(int device, void** table, void** lists);
 
 }  // namespace xpbd

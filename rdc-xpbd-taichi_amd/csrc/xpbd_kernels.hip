@@ -19,6 +19,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <mutex>
+
 namespace xpbd {
 
 namespace {
@@ -946,6 +948,44 @@ __global__ void k_check_finite(const float4* __restrict__ pos, int n, int* flag)
   if (i >= n) return;
   const float4 a = pos[i];
   if (!isfinite(a.x) || !isfinite(a.y) || !isfinite(a.z)) atomicOr(flag, 1);
+}
+
+// ---- the no-allocation work-list array ------------------------------------------
+
+// Work lists and scene table of a batch re-targeted without allocating
+// (xpbd_batch_set_members_noalloc): a fixed array in device memory, one copy
+// per GPU, held until the process ends.  Laid out as the scene table
+// (XPBD_NOALLOC_MAX_SCENES entries) followed by XPBD_NOALLOC_LIST_BYTES of
+// work lists.
+constexpr size_t kNoallocTableBytes =
+    (sizeof(SceneDev) * XPBD_NOALLOC_MAX_SCENES + 255) / 256 * 256;
+constexpr size_t kNoallocBytes = kNoallocTableBytes + XPBD_NOALLOC_LIST_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void** table, void** lists) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    // once per device, with that device current: the array's address on this
+    // GPU (the host-side name of the array is not a device address)
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *table = g_scratch_ptr[device];
+  *lists = static_cast<unsigned char*>(g_scratch_ptr[device]) + kNoallocTableBytes;
+  return hipSuccess;
 }
 
 // ---- launch wrappers ---------------------------------------------------------

@@ -38,6 +38,8 @@ LAYOUT_PATCH = 1
 ORDER_NONE = 0
 ORDER_BISECT = 1
 ORDER_MORTON = 2
+NOALLOC_MAX_SCENES = 64
+NOALLOC_LIST_BYTES = 1 << 20
 
 # Every symbol the header declares; tests check that the library exports them.
 EXPORTED = (
@@ -52,7 +54,8 @@ EXPORTED = (
     "xpbd_get_state", "xpbd_set_state", "xpbd_get_sizes", "xpbd_set_exec_path",
     "xpbd_set_kernel_timing", "xpbd_get_kernel_time", "xpbd_check_finite", "xpbd_last_error",
     "xpbd_patch_stats", "xpbd_get_device_bytes", "xpbd_batch_opts_init", "xpbd_batch_create",
-    "xpbd_batch_destroy", "xpbd_batch_set_members", "xpbd_batch_substep", "xpbd_batch_frame",
+    "xpbd_batch_destroy", "xpbd_batch_set_members", "xpbd_batch_set_members_noalloc",
+    "xpbd_batch_substep", "xpbd_batch_frame",
     "xpbd_batch_synchronize", "xpbd_batch_get_stream", "xpbd_batch_info", "xpbd_batch_grid",
 )
 
@@ -200,6 +203,7 @@ def load() -> ctypes.CDLL:
                               pp],
         "xpbd_batch_destroy": [vp],
         "xpbd_batch_set_members": [vp, ctypes.POINTER(ctypes.c_void_p), i32],
+        "xpbd_batch_set_members_noalloc": [vp, ctypes.POINTER(ctypes.c_void_p), i32],
         "xpbd_batch_substep": [vp],
         "xpbd_batch_frame": [vp, i32],
         "xpbd_batch_synchronize": [vp],
