@@ -198,7 +198,19 @@ def committed_traffic(path: str = PMC_SUMMARY):
     return None
 
 
-def cpu_baseline(scene, substeps: int, threads: int) -> dict:
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _cpu_leg(scene, substeps: int, threads: int) -> dict:
     from oracle import Oracle  # the CPU restatement, used here only as the baseline
     p = dict(scene.params)
     o = Oracle(scene.mesh.verts, scene.mesh.tets, scene.sdfs, precision=32, threads=threads,
@@ -211,10 +223,30 @@ def cpu_baseline(scene, substeps: int, threads: int) -> dict:
         o.substep(sub_dt)
     el = time.perf_counter() - t0
     tit = scene.mesh.n_tets * p["constraints_iter"] * substeps
-    return {"value": tit / el, "unit": "tet-iterations/s", "cores": threads, "kind": "port",
+    return {"tet_iterations_per_s": tit / el, "seconds": el, "tet_iterations": tit,
+            "substeps": substeps}
+
+
+def cpu_baseline(scene, substeps: int, threads: int) -> dict:
+    """The CPU restatement of the reference kernel sequence (the f32 oracle,
+    OpenMP over tets, `threads` threads) on the GPU box's host: a bounded
+    sample of the headline mesh (`substeps` substeps of the 64^3 box) and, as
+    a second leg, the 16^3 cube's 10 frames in full."""
+    from xpbd_gpu.scenes import resting_cube
+    big = _cpu_leg(scene, substeps, threads)
+    cube = resting_cube()
+    small = _cpu_leg(cube, 10 * cube.params["substeps_num"], threads)
+    p = scene.params
+    return {"value": big["tet_iterations_per_s"], "unit": "tet-iterations/s", "cores": threads,
+            "kind": "port", "host_nproc": os.cpu_count(), "cpu_model": _cpu_model(),
             "sample": f"{substeps} substeps x {p['constraints_iter']} iterations of the same "
-                      f"mesh ({tit / 1e6:.0f} M tet-iterations, {el:.1f} s), f32 oracle, "
-                      f"OpenMP over tets"}
+                      f"64^3 mesh ({big['tet_iterations'] / 1e6:.0f} M tet-iterations, "
+                      f"{big['seconds']:.1f} s), f32 oracle (CPU restatement of the reference "
+                      f"kernel sequence), OpenMP over tets on {threads} threads",
+            "small_leg": {"workload": "16^3 cube resting on the plane, 10 frames of 5 x 30 "
+                                      "iterations (24 576 tets)",
+                          "tet_iterations_per_s": small["tet_iterations_per_s"],
+                          "seconds": small["seconds"]}}
 
 
 def batched_measurement(n_scenes: int, frames: int, tune: bool, sequential: bool = True) -> dict:
@@ -355,13 +387,20 @@ def small_configs_measurement(frames: int = 10) -> dict:
     falls from 0.5, and the same cube resting on the plane is the contact
     case) and the 32^3 twisted grid (196 608 tets, one substep of 200
     iterations per frame, relaxing from its stretched and twisted start).
-    Each runs one untimed frame, then `frames` timed frames."""
+    Each runs one untimed frame, then `frames` timed frames, on the default
+    execution path ("auto": the persistent substep kernel for the cube, the
+    graph of phased kernels for the grid) and, for comparison, on the graph
+    path."""
     from xpbd_gpu.scenes import dropped_cube, make_solver, resting_cube, twisted_grid
 
     out = {}
+    runs = []
     for name, scene in (("dropped_cube16", dropped_cube()), ("resting_cube16", resting_cube()),
                         ("twisted_grid32", twisted_grid())):
-        solver = make_solver(scene)
+        runs.append((name, scene, "auto"))
+        runs.append((name + "_graph", scene, "graph"))
+    for name, scene, path in runs:
+        solver = make_solver(scene, exec_path=path)
         T, V = scene.mesh.n_tets, scene.mesh.n_verts
         it_per_frame = scene.params["substeps_num"] * scene.params["constraints_iter"]
         solver.solve()
@@ -374,7 +413,8 @@ def small_configs_measurement(frames: int = 10) -> dict:
             lat.append(time.perf_counter() - s0)
         el = time.perf_counter() - t0
         solver.check_finite()
-        out[name] = {"tets": T, "verts": V, "iterations_per_frame": it_per_frame,
+        out[name] = {"tets": T, "verts": V, "exec_path": path,
+                     "iterations_per_frame": it_per_frame,
                      "frames": frames, "ms_per_frame": el / frames * 1e3,
                      "frame_p99_ms": float(np.percentile(lat, 99) * 1e3),
                      "tet_iterations_per_s": T * it_per_frame * frames / el,
@@ -393,7 +433,8 @@ def main():
     ap.add_argument("--n", type=int, default=64, help="hexes per axis")
     ap.add_argument("--accumulate", default="csr", choices=["atomic", "csr"])
     ap.add_argument("--layout", default="patch", choices=["patch", "flat"])
-    ap.add_argument("--exec-path", default="graph", choices=["graph", "kernels"])
+    ap.add_argument("--exec-path", default="auto",
+                    choices=["auto", "graph", "persistent", "kernels"])
     ap.add_argument("--cpu-substeps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -551,6 +592,7 @@ def main():
         out["served"] = served_measurement(args.serve_scenes, args.serve_rate,
                                            args.serve_seconds)
     if not args.no_cpu_baseline:
+        # the box's CPU share is 16 threads per GPU, whatever nproc shows
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(scene, args.cpu_substeps, threads)
     print(json.dumps(out), flush=True)

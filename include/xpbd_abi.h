@@ -204,9 +204,25 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets);
 
 /* Execution-path selection (no reference counterpart).
  *  XPBD_PATH_KERNELS:    one launch per kernel per iteration.
- *  XPBD_PATH_GRAPH:      the same kernels, each substep captured in a hipGraph. */
+ *  XPBD_PATH_GRAPH:      the same kernels, each substep captured in a hipGraph.
+ *  XPBD_PATH_PERSISTENT: for small scenes (at most 256 patches of 512 tets,
+ *                        the patch layout with Chebyshev on and no isolated
+ *                        vertices, at least one iteration), one launch per
+ *                        substep that runs every iteration with scene-wide
+ *                        barriers between its phases; bit-identical to the
+ *                        other paths in deterministic mode.  Other scenes take
+ *                        the graph path.  If a barrier cannot complete (the
+ *                        kernel's workgroups not all resident, e.g. beside
+ *                        another persistent kernel), it gives up after 2 s and
+ *                        the next synchronizing call reports XPBD_EHIP. */
 #define XPBD_PATH_KERNELS 0
 #define XPBD_PATH_GRAPH 1
+#define XPBD_PATH_PERSISTENT 2
+/*  XPBD_PATH_AUTO:       the persistent substep for scenes of at most 96
+ *                        patches (where its scene barriers cost less than the
+ *                        phased launches' dependent loads), the graph path
+ *                        otherwise. */
+#define XPBD_PATH_AUTO 3
 int xpbd_set_exec_path(xpbd_solver* s, int32_t path);
 
 /* Kernel timing (no reference counterpart).  mode 0: off.  mode 1: every

@@ -96,6 +96,8 @@ struct SceneDev {
                          // component c at [c * n_partials])
   gptr<SceneCtrl> ctrl;   // two slots, see the controller update in the kernels
   gptr<double> log;      // 6 doubles per iteration: iter, r_hat, rho_hat, omega, hydro, dev
+  // persistent substep: grid-barrier arrivals, workgroups done, error word
+  gptr<unsigned> sync;
   int n_verts;
   int n_tets;
   int n_partials;
@@ -157,6 +159,10 @@ constexpr int kPatchVerts = 256;
 // a scene's sums do not depend on how work is split or which scenes share a
 // batch.  The residual kernel's work ranges start at group boundaries.
 constexpr int kPartialGroup = 2;
+
+// Persistent substep kernel: one workgroup of kPersistThreads threads per
+// residual group of kPartialGroup patches (one patch per half).
+constexpr int kPersistThreads = kPatchThreads * kPartialGroup;
 
 // Reciprocal and square root on the native gfx950 instructions (v_rcp_f32,
 // v_sqrt_f32: about 1 ulp) instead of the correctly rounded sequences (a

@@ -431,6 +431,12 @@ struct xpbd_solver {
   std::vector<PatchItem> pitems_host;
   DevBuf<int2> prange_proj, prange_res;
   int grid_proj = 0, grid_res = 0;
+  // persistent substep: barrier words (arrivals, workgroups done, error), the
+  // most workgroups of its kernel that can be resident at once, and whether it
+  // has run since the error word was last read
+  DevBuf<unsigned> sync;
+  int persist_capacity = 0;
+  bool persist_pending = false;
   int n_patches = 0;
   int layout = XPBD_LAYOUT_PATCH;
   DevBuf<float> valence;
@@ -478,7 +484,7 @@ struct xpbd_solver {
     add(vslot_off); add(p_vert_sum); add(tet_local); add(pitems); add(prange_proj);
     add(prange_res); add(valence); add(adj_off); add(adj); add(d_old2new); add(tet); add(dm0);
     add(dm1); add(dm2); add(lam); add(partial); add(logbuf); add(ctrl); add(table); add(vmap);
-    add(tmap); add(scratch_xyz); add(flag); add(lam_save);
+    add(tmap); add(scratch_xyz); add(flag); add(lam_save); add(sync);
     return b;
   }
 
@@ -498,6 +504,24 @@ struct xpbd_solver {
 };
 
 namespace {
+
+// Waits for the handle's stream.  When the persistent substep has run since
+// the last wait, its error word is read: a set word (its workgroups were not
+// all resident, so a scene barrier gave up) is cleared and reported.
+int stream_sync(xpbd_solver* s) {
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (!s->persist_pending) return XPBD_OK;
+  s->persist_pending = false;
+  unsigned err = 0;
+  HIP_TRY(hipMemcpy(&err, s->sync.p + 2, sizeof(err), hipMemcpyDeviceToHost));
+  if (err) {
+    const unsigned zero = 0;
+    HIP_TRY(hipMemcpy(s->sync.p + 2, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    return set_error(XPBD_EHIP, "persistent substep: a scene barrier timed out (its workgroups "
+                                "were not all resident); the substep's results are invalid");
+  }
+  return XPBD_OK;
+}
 
 int validate_params(const xpbd_desc* d) {
   if (d->substeps_num < 1) return set_error(XPBD_EINVAL, "substeps_num must be >= 1");
@@ -576,6 +600,7 @@ int build_scene(xpbd_solver* s) {
   S.partial = s->partial.p;
   S.ctrl = s->ctrl.p;
   S.log = s->logbuf.p;
+  S.sync = s->sync.p;
   S.n_verts = s->nv;
   S.n_tets = s->nt;
   S.layout = s->layout;
@@ -923,7 +948,7 @@ int fetch_log(xpbd_solver* s) {
   if (iters == 0) return XPBD_OK;
   HIP_TRY(hipMemcpyAsync(s->log_host, s->logbuf.p, sizeof(double) * 6 * iters,
                          hipMemcpyDeviceToHost, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   for (int it = 0; it < iters; ++it) {
     const double* r = s->log_host + 6 * it;
     const double row[8] = {double(s->frame), double(s->substep), r[0], r[1], r[2], r[3], r[4], r[5]};
@@ -932,9 +957,39 @@ int fetch_log(xpbd_solver* s) {
   return XPBD_OK;
 }
 
+// Persistent substep limits: the wait of one scene barrier (ticks of the
+// 100 MHz wall clock) and the largest scene it is used for.
+constexpr unsigned long long kPersistTimeoutTicks = 200000000ull;  // 2 s
+constexpr int kPersistMaxGroups = 128;                             // 256 patches
+// XPBD_PATH_AUTO takes the persistent substep up to this many groups: a scene
+// barrier costs 1.6 us at 8 workgroups, 2.6 us at 24, 3.8 us at 48 (agent-scope
+// release and acquire per workgroup), and two per iteration beat the phased
+// launches up to about this size.
+constexpr int kAutoPersistGroups = 48;
+
+// Whether the persistent substep kernel can run this scene: the fused
+// sequence with at least one iteration, and few enough groups to be resident
+// together.
+bool persistent_ok(const xpbd_solver* s) {
+  const SceneDev& S = s->host_scene;
+  const int groups = S.n_partials;
+  return s->layout == XPBD_LAYOUT_PATCH && S.fused && S.iters >= 1 && groups >= 1 &&
+         groups <= kPersistMaxGroups && groups <= s->persist_capacity;
+}
+
 int run_substep(xpbd_solver* s, double sub_dt) {
   TRY(sync_table(s, sub_dt, s->prm.damping));
-  if (s->path == XPBD_PATH_GRAPH && !s->timing) {
+  const bool persist = (s->path == XPBD_PATH_PERSISTENT ||
+                        (s->path == XPBD_PATH_AUTO &&
+                         s->host_scene.n_partials <= kAutoPersistGroups)) &&
+                       !s->timing && persistent_ok(s);
+  if (persist) {
+    HIP_TRY(xpbd::launch_substep_persistent(s->table.p, s->pitems.p, s->host_scene.n_partials,
+                                            kPersistTimeoutTicks, s->stream));
+    s->persist_pending = true;
+    return fetch_log(s);
+  }
+  if (s->path != XPBD_PATH_KERNELS && !s->timing) {
     const int key = graph_key_of(s->host_scene);
     if (!s->gexec || key != s->graph_key) {
       if (s->gexec) {
@@ -1209,7 +1264,12 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
     TRY(grouped_ranges(items, s->grid_res, xcd_count(), rr, &s->grid_res));
     TRY(s->prange_proj.upload(rp.data(), rp.size()));
     TRY(s->prange_res.upload(rr.data(), rr.size()));
+    int occ_s = 0;
+    HIP_TRY(xpbd::persistent_occupancy(&occ_s));
+    s->persist_capacity = n_cu * occ_s;
   }
+  TRY(s->sync.alloc(4));
+  TRY(s->sync.zero());
   TRY(s->lam.alloc(nt));
   TRY(s->lam.zero());
   TRY(s->d_old2new.upload(s->v_old2new.data(), nv));
@@ -1278,7 +1338,7 @@ int xpbd_update_params(xpbd_solver* s, const xpbd_desc* d) {
   s->prm.reorder = keep.reorder;
   s->prm.device = keep.device;
   if (d->constraints_iter > old_iters) {
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    TRY(stream_sync(s));
     TRY(s->logbuf.alloc(6 * size_t(d->constraints_iter)));
     TRY(s->logbuf.zero());
     if (s->log_host) (void)hipHostFree(s->log_host);
@@ -1322,7 +1382,7 @@ int xpbd_get_positions(xpbd_solver* s, float* xyz) {
   HIP_TRY(xpbd::launch_gather_xyz(s->scratch_xyz.p, s->xs.p, s->d_old2new.p, s->nv, s->stream));
   HIP_TRY(hipMemcpyAsync(xyz, s->scratch_xyz.p, sizeof(float) * 3 * s->nv, hipMemcpyDeviceToHost,
                          s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   return XPBD_OK;
 }
 
@@ -1331,14 +1391,14 @@ int xpbd_get_positions_device(xpbd_solver* s, void* dev_xyz) {
   if (!dev_xyz) return set_error(XPBD_EINVAL, "NULL buffer");
   HIP_TRY(xpbd::launch_gather_xyz(static_cast<float*>(dev_xyz), s->xs.p, s->d_old2new.p, s->nv,
                                   s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   return XPBD_OK;
 }
 
 int xpbd_set_positions(xpbd_solver* s, const float* xyz) {
   CHECK_HANDLE(s);
   if (!xyz) return set_error(XPBD_EINVAL, "NULL buffer");
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   HIP_TRY(hipMemcpy(s->scratch_xyz.p, xyz, sizeof(float) * 3 * s->nv, hipMemcpyHostToDevice));
   return set_positions_common(s, s->scratch_xyz.p);
 }
@@ -1355,17 +1415,17 @@ int xpbd_get_velocities(xpbd_solver* s, float* xyz) {
   HIP_TRY(xpbd::launch_gather_xyz(s->scratch_xyz.p, s->vel.p, s->d_old2new.p, s->nv, s->stream));
   HIP_TRY(hipMemcpyAsync(xyz, s->scratch_xyz.p, sizeof(float) * 3 * s->nv, hipMemcpyDeviceToHost,
                          s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   return XPBD_OK;
 }
 
 int xpbd_set_velocities(xpbd_solver* s, const float* xyz) {
   CHECK_HANDLE(s);
   if (!xyz) return set_error(XPBD_EINVAL, "NULL buffer");
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   HIP_TRY(hipMemcpy(s->scratch_xyz.p, xyz, sizeof(float) * 3 * s->nv, hipMemcpyHostToDevice));
   HIP_TRY(xpbd::launch_scatter_xyz(s->vel.p, s->scratch_xyz.p, s->d_old2new.p, s->nv, 0, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   return XPBD_OK;
 }
 
@@ -1375,7 +1435,7 @@ int xpbd_get_predicted(xpbd_solver* s, float* xyz) {
   HIP_TRY(xpbd::launch_gather_xyz(s->scratch_xyz.p, s->pos.p, s->d_old2new.p, s->nv, s->stream));
   HIP_TRY(hipMemcpyAsync(xyz, s->scratch_xyz.p, sizeof(float) * 3 * s->nv, hipMemcpyDeviceToHost,
                          s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   return XPBD_OK;
 }
 
@@ -1399,7 +1459,7 @@ int xpbd_get_degenerate_tets(xpbd_solver* s, int32_t* count, int32_t* first) {
 int xpbd_get_multipliers(xpbd_solver* s, float* lambda_d, float* lambda_h) {
   CHECK_HANDLE(s);
   std::vector<float2> l(s->nt);
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   HIP_TRY(hipMemcpy(l.data(), s->lam.p, sizeof(float2) * s->nt, hipMemcpyDeviceToHost));
   for (int t = 0; t < s->nt; ++t) {
     const int o = s->t_new2old[t];
@@ -1476,7 +1536,7 @@ int xpbd_frame(xpbd_solver* s, int32_t n_frames) {
 
 int xpbd_synchronize(xpbd_solver* s) {
   CHECK_HANDLE(s);
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   return XPBD_OK;
 }
 
@@ -1519,7 +1579,7 @@ int xpbd_get_state(xpbd_solver* s, int32_t* frame, int32_t* substep, double* tim
   if (time) *time = s->time;
   if (rho_hat) {
     SceneCtrl c;
-    HIP_TRY(hipStreamSynchronize(s->stream));
+    TRY(stream_sync(s));
     // between substeps both controller slots hold the same state
     HIP_TRY(hipMemcpy(&c, s->ctrl.p, sizeof(c), hipMemcpyDeviceToHost));
     *rho_hat = c.rho_hat;
@@ -1530,7 +1590,7 @@ int xpbd_get_state(xpbd_solver* s, int32_t* frame, int32_t* substep, double* tim
 int xpbd_set_state(xpbd_solver* s, int32_t frame, double time, double rho_hat) {
   CHECK_HANDLE(s);
   SceneCtrl c;
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   HIP_TRY(hipMemcpy(&c, s->ctrl.p, sizeof(c), hipMemcpyDeviceToHost));
   if (!std::isnan(rho_hat)) {
     c.rho_hat = rho_hat;
@@ -1553,7 +1613,8 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets) {
 
 int xpbd_set_exec_path(xpbd_solver* s, int32_t path) {
   if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
-  if (path != XPBD_PATH_KERNELS && path != XPBD_PATH_GRAPH)
+  if (path != XPBD_PATH_KERNELS && path != XPBD_PATH_GRAPH && path != XPBD_PATH_PERSISTENT &&
+      path != XPBD_PATH_AUTO)
     return set_error(XPBD_EINVAL, "unknown exec path %d", path);
   s->path = path;
   return XPBD_OK;
@@ -1665,7 +1726,7 @@ int xpbd_check_finite(xpbd_solver* s) {
   HIP_TRY(xpbd::launch_check_finite(s->xs.p, s->nv, s->flag.p, s->stream));
   HIP_TRY(xpbd::launch_check_finite(s->pos.p, s->nv, s->flag.p, s->stream));
   HIP_TRY(hipMemcpyAsync(&hit, s->flag.p, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-  HIP_TRY(hipStreamSynchronize(s->stream));
+  TRY(stream_sync(s));
   if (hit) return set_error(XPBD_ENUMERIC, "non-finite vertex position in the solver state");
   return XPBD_OK;
 }

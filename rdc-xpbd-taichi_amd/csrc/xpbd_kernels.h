@@ -34,6 +34,11 @@ hipError_t launch_blend(const SceneDev* scenes, const WorkItem* vmap, int nblock
                         hipStream_t st);
 hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                           hipStream_t st);
+// One substep of one scene in one launch (n_groups workgroups, all resident);
+// a scene barrier gives up after timeout_ticks of the 100 MHz wall clock.
+hipError_t launch_substep_persistent(const SceneDev* scenes, const PatchItem* items, int n_groups,
+                                    unsigned long long timeout_ticks, hipStream_t st);
+hipError_t persistent_occupancy(int* blocks_per_cu);
 hipError_t launch_scatter_xyz(float4* dst, const float* src, const int* old_to_new, int n,
                               int keep_w, hipStream_t st);
 hipError_t launch_gather_xyz(float* dst, const float4* src, const int* old_to_new, int n,

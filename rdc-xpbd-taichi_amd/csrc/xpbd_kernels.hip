@@ -198,12 +198,10 @@ __device__ __forceinline__ void store_partial(const SceneDev& S, int idx, const 
   S.partial[2 * n + idx] = tot[2];
 }
 
-// x_J = y + delta_x / cons_num for one vertex (post_solve,
-// neohookean_solver.py:115-119), with delta_x summed from the vertex's patch
-// slots in ascending patch order.
-__device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v, int it) {
+// delta_x (xyz) and the constraint count (w) of vertex v summed over its slots
+// [b, e) in ascending patch order.
+__device__ __forceinline__ float4 slot_sum(const SceneDev& S, int b, int e) {
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int b = S.vslot_off[v], e = S.vslot_off[v + 1];
   int a = b;
   // four slot loads in flight, added in the same order as one at a time
   for (; a + 4 <= e; a += 4) {
@@ -221,14 +219,59 @@ __device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v,
     acc.z += c.z;
     acc.w += c.w;
   }
-  const float cnt = S.block_mode ? S.valence[v] : acc.w;
-  const float4 p = S.fused ? S.ybuf[it % 3][v] : S.pos[v];
+  return acc;
+}
+
+// x_J = y + delta_x / cons_num (post_solve, neohookean_solver.py:115-119);
+// cons_num is the valence in block mode, the summed count in split mode.
+__device__ __forceinline__ float4 jacobi_from(const SceneDev& S, float4 acc, float valence,
+                                              float4 p) {
+  const float cnt = S.block_mode ? valence : acc.w;
   float3 x = f3(p);
   if (cnt > 0.f) {
     const float ic = frcp(cnt);
     x = x + make_float3(acc.x * ic, acc.y * ic, acc.z * ic);
   }
   return make_float4(x.x, x.y, x.z, p.w);
+}
+
+// x_J of vertex v with delta_x summed from its patch slots.
+__device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v, int it) {
+  const float4 acc = slot_sum(S, S.vslot_off[v], S.vslot_off[v + 1]);
+  const float4 p = S.fused ? S.ybuf[it % 3][v] : S.pos[v];
+  return jacobi_from(S, acc, S.valence[v], p);
+}
+
+// apply_external_forces (xpbd_base.py:139-146) and the collider projections
+// of solve_sdf_collision (xpbd_base.py:162-179) for one vertex: from x (xs),
+// v and inv_m (w of pos) to the predicted position; vel is updated.
+__device__ __forceinline__ float3 predict_vertex(const SceneDev& S, float3 xs, float3& vel,
+                                                 float w, bool external, bool collide,
+                                                 float3 x) {
+  if (external) {
+    const float dt = S.dt;
+    if (w > 0.f) vel = vel + make_float3(S.g[0], S.g[1], S.g[2]) * dt;
+    x = xs + vel * dt;
+  }
+  if (collide && w > 0.f) {
+    for (int s = 0; s < S.n_sdf; ++s) x = sdf_project(S.sdf[s], x, vel);
+  }
+  return x;
+}
+
+// advance (xpbd_base.py:148-160) of one vertex with final position p and start
+// position x: pinned vertices go back to x; free ones get v = (p - x) / dt *
+// exp(-dt damping) and x = p.  Writes pos (pinned) or vel and xs (free).
+__device__ __forceinline__ void advance_vertex(const SceneDev& S, int v, float4 p, float4 x) {
+  if (p.w <= 0.f) {
+    S.pos[v] = make_float4(x.x, x.y, x.z, p.w);
+  } else {
+    const float dt = S.dt;
+    const float3 raw = make_float3((p.x - x.x) / dt, (p.y - x.y) / dt, (p.z - x.z) / dt);
+    const float3 vel = raw * S.decay;
+    S.vel[v] = make_float4(vel.x, vel.y, vel.z, 0.f);
+    S.xs[v] = make_float4(p.x, p.y, p.z, x.w);
+  }
 }
 
 }  // namespace
@@ -243,18 +286,11 @@ __global__ void k_predict(const SceneDev* __restrict__ scenes, const WorkItem* _
   const int v = wi.first + threadIdx.x;
   if (v >= S.n_verts) return;
   float4 p = S.pos[v];
-  float3 x = f3(p);
   float3 vel = f3(S.vel[v]);
   const float w = p.w;
-  if (external) {
-    const float dt = S.dt;
-    if (w > 0.f) vel = vel + make_float3(S.g[0], S.g[1], S.g[2]) * dt;
-    x = f3(S.xs[v]) + vel * dt;
-    S.vel[v] = make_float4(vel.x, vel.y, vel.z, 0.f);
-  }
-  if (collide && w > 0.f) {
-    for (int s = 0; s < S.n_sdf; ++s) x = sdf_project(S.sdf[s], x, vel);
-  }
+  const float3 x = predict_vertex(S, external ? f3(S.xs[v]) : f3(p), vel, w, external != 0,
+                                  collide != 0, f3(p));
+  if (external) S.vel[v] = make_float4(vel.x, vel.y, vel.z, 0.f);
   p = make_float4(x.x, x.y, x.z, w);
   S.pos[v] = p;
   if (save_prev) S.prev[v] = p;
@@ -544,11 +580,12 @@ struct PatchTets {
   float2 lam[kPatchTetsPerThread];
 };
 
+// `lt` is the thread's index among the kPatchThreads threads working on h.
 __device__ __forceinline__ void load_patch_tets(const SceneDev& S, const PatchItem& h, bool lam,
-                                                PatchTets& d) {
+                                                PatchTets& d, int lt = threadIdx.x) {
 #pragma unroll
   for (int j = 0; j < kPatchTetsPerThread; ++j) {
-    const int t = threadIdx.x + j * kPatchThreads;
+    const int t = lt + j * kPatchThreads;
     d.loc[j] = make_uint2(0u, 0u);
     d.lam[j] = make_float2(0.f, 0.f);
     if (t < h.n_tets) {
@@ -908,16 +945,342 @@ __global__ void k_advance(const SceneDev* __restrict__ scenes, const WorkItem* _
   const SceneDev& S = scenes[wi.scene];
   const int v = wi.first + threadIdx.x;
   if (v >= S.n_verts) return;
-  const float4 p = S.pos[v];
-  const float4 x = S.xs[v];
-  if (p.w <= 0.f) {
-    S.pos[v] = make_float4(x.x, x.y, x.z, p.w);
-  } else {
-    const float dt = S.dt;
-    const float3 raw = make_float3((p.x - x.x) / dt, (p.y - x.y) / dt, (p.z - x.z) / dt);
-    const float3 vel = raw * S.decay;
-    S.vel[v] = make_float4(vel.x, vel.y, vel.z, 0.f);
-    S.xs[v] = make_float4(p.x, p.y, p.z, x.w);
+  advance_vertex(S, v, S.pos[v], S.xs[v]);
+}
+
+// ---- persistent substep (small scenes) -------------------------------------------
+
+// A scene-wide barrier over the persistent substep's workgroups.  The
+// workgroup barrier orders every thread's global writes before thread 0
+// (work-group scope), which releases them at agent scope, counts the
+// workgroup in and waits, without blocking forever, until `target` arrivals
+// are counted; it then acquires at agent scope, and the closing workgroup
+// barrier passes that on to the other threads.  (One release and one acquire
+// per workgroup: each is an L2 write-back or invalidate on this GPU.)  The
+// wait ends early when another workgroup has set the error word, and after
+// `timeout` ticks of the 100 MHz wall clock it sets the error word itself
+// (the workgroups are not all resident).  Returns false on either: the
+// caller then leaves its loops.
+__device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, unsigned target,
+                                              unsigned long long timeout, int* s_ok) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int ok = 1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t0 = wall_clock64();
+    while (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+        ok = 0;
+        break;
+      }
+      if (wall_clock64() - t0 > timeout) {
+        __hip_atomic_fetch_or(&sync[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    *s_ok = ok;
+  }
+  __syncthreads();
+  return *s_ok != 0;
+}
+
+// scene_sums by the first kReduceThreads threads of a larger workgroup (the
+// others only meet the barriers), in the same order, so the same bits.
+__device__ __forceinline__ void scene_sums_first(const SceneDev& S, bool all3, double tot[3],
+                                                 double* red) {
+  constexpr int kW = kReduceThreads / 64;
+  const int n = S.n_partials;
+  const int tid = threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (tid < kReduceThreads) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if (c > 0 && !all3) break;
+      gptr<const double> p = S.partial + size_t(c) * size_t(n);
+      double sum = 0.0;
+      for (int b = tid; b < n; b += 8 * kReduceThreads) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = p[min(b + k * kReduceThreads, n - 1)];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (b + k * kReduceThreads < n) sum += v[k];
+      }
+      acc[c] = wave_sum_d(sum);
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+    if (lane == 0) {
+      red[wave] = acc[0];
+      red[kW + wave] = acc[1];
+      red[2 * kW + wave] = acc[2];
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    tot[0] = tot[1] = tot[2] = 0.0;
+    for (int w = 0; w < kW; ++w) {
+      tot[0] += red[w];
+      tot[1] += red[kW + w];
+      tot[2] += red[2 * kW + w];
+    }
+  }
+  __syncthreads();
+}
+
+// One whole substep of one scene (predict, every Jacobi iteration with its
+// rho/omega update and Chebyshev blend, the final blend and advance) in one
+// launch, for scenes of few patches, whose phased launches are bound by
+// latency.  Workgroup g holds residual group g: patches 2g and 2g + 1, one per
+// half of its kPersistThreads threads, each half working like a workgroup of
+// the phased kernels.  The iterate, its predecessor, the Jacobi iterate and
+// the rest positions of the group's vertices stay in LDS, and the
+// multipliers in registers, for the whole substep; each patch predicts and
+// blends its own copies of its vertices (the same bits in every patch that
+// holds a vertex, as in the phased kernels).  Two scene barriers per
+// iteration: after the slot partials are written, and after the residual
+// partials.  Every sum is taken in the phased kernels' order, so the results
+// are bit-identical to them in deterministic mode.  Requires the fused
+// sequence (patch layout, Chebyshev on, no isolated vertices), at least one
+// iteration, and all n_groups workgroups resident at once.
+__global__ void __launch_bounds__(kPersistThreads)
+    k_substep_persistent(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
+                         int n_groups, unsigned long long timeout) {
+  const SceneDev& S = scenes[0];
+  const int tid = threadIdx.x;
+  const int half = tid / kPatchThreads, lt = tid % kPatchThreads;
+  const int p = blockIdx.x * kPartialGroup + half;
+  const bool active = p < S.n_patches;
+  const bool csr = S.csr != 0;
+  const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+  __shared__ float4 s_y[kPartialGroup][kPatchVerts];     // y^k of the patch's vertices
+  __shared__ float4 s_prev[kPartialGroup][kPatchVerts];  // y^(k-1) (y^0 at k = 0, 1)
+  __shared__ float4 s_x[kPartialGroup][kPatchVerts];     // x_J^k
+  __shared__ float4 s_rest[kPartialGroup][kPatchVerts];  // rest position | corner run base
+  __shared__ float4 s_sorted[kPartialGroup][4 * kPatchTets];
+  __shared__ double s_red[kReduceScratch];
+  __shared__ float s_omega;
+  __shared__ int s_ok;
+  __shared__ SceneCtrl s_ctrl;
+  gptr<unsigned> sync = S.sync;
+  float4* my_sorted = s_sorted[half];
+
+  // the patch's vertices: ids, LDS corner runs, slot ranges; the thread's tets
+  PatchItem h;
+  h.n_tets = 0;
+  h.n_verts = 0;
+  if (active) h = items[p];
+  int g = 0, sb = 0, se = 0;
+  int2 vs = make_int2(0, 0);
+  float val = 0.f;
+  if (lt < h.n_verts) {
+    g = S.p_vert_ids[h.vert_begin + lt];
+    vs = S.p_vert_sum[h.vert_begin + lt];
+    sb = S.vslot_off[g];
+    se = S.vslot_off[g + 1];
+    val = S.valence[g];
+  }
+  PatchTets t;
+  load_patch_tets(S, h, false, t, lt);  // the multipliers start at 0 (:458-459)
+  // predict + collide (each patch for its own copy of its vertices)
+  float4 xs0 = zero;
+  if (lt < h.n_verts) {
+    const float4 pw = S.pos[g];
+    xs0 = S.xs[g];
+    float3 vel = f3(S.vel[g]);
+    const float3 x = predict_vertex(S, f3(xs0), vel, pw.w, true, true, f3(pw));
+    const float4 y0 = make_float4(x.x, x.y, x.z, pw.w);
+    s_y[half][lt] = y0;
+    s_prev[half][lt] = y0;
+    const float4 r = S.rest[g];
+    s_rest[half][lt] = make_float4(r.x, r.y, r.z, __int_as_float(vs.x & 0xffff));
+  }
+  if (tid == 0) s_ctrl = S.ctrl[0];  // between substeps both slots hold the same state
+  __syncthreads();
+  // rest shape of the thread's tets, kept in registers for the substep
+  Mat3 idm[kPatchTetsPerThread];
+  float inv_det[kPatchTetsPerThread];
+#pragma unroll
+  for (int j = 0; j < kPatchTetsPerThread; ++j) {
+    const uint2 loc = t.loc[j];
+    const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu), int((loc.x >> 16) & 0xffu),
+                      int(loc.x >> 24)};
+    rest_shape(f3(s_rest[half][l[0]]), f3(s_rest[half][l[1]]), f3(s_rest[half][l[2]]),
+               f3(s_rest[half][l[3]]), idm[j], inv_det[j]);
+  }
+  const unsigned G = unsigned(n_groups);
+  unsigned barriers = 0;
+  bool ok = true;
+  const bool writer = blockIdx.x == 0;
+  for (int it = 0; it < S.iters && ok; ++it) {
+    if (it > 0) {
+      // rho/omega update after iteration it - 1 from every group's partial
+      double tot[3];
+      scene_sums_first(S, S.monitor != 0, tot, s_red);
+      if (tid == 0) {
+        double L[6];
+        const SceneCtrl c = ctrl_update(S, s_ctrl, it - 1, tot, L);
+        s_ctrl = c;
+        s_omega = float(c.omega);
+        if (writer) {
+          S.ctrl[(it - 1) & 1] = c;
+          store_log_row(S, it - 1, L);
+        }
+      }
+      __syncthreads();
+      if (lt < h.n_verts) {  // y^it from y^(it-1), x_J^(it-1) and y^(it-2)
+        const float4 y = s_y[half][lt];
+        s_y[half][lt] = cheb_blend(S, y, s_x[half][lt], s_prev[half][lt], s_omega);
+        s_prev[half][lt] = y;
+      }
+      __syncthreads();
+    }
+    // projection: corner corrections into LDS, summed per vertex into its slot
+    if (!csr && lt < h.n_verts) my_sorted[lt] = zero;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPatchTetsPerThread; ++j) {
+      const int tt = lt + j * kPatchThreads;
+      if (tt < h.n_tets) {
+        const uint2 loc = t.loc[j];
+        const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu),
+                          int((loc.x >> 16) & 0xffu), int(loc.x >> 24)};
+        const float4 q[4] = {s_y[half][l[0]], s_y[half][l[1]], s_y[half][l[2]], s_y[half][l[3]]};
+        TetProjection P;
+        project_tet(S, q, idm[j], inv_det[j], t.lam[j], P);
+        if (csr) {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            const int rank = int((loc.y >> (8 * k)) & 0xffu);
+            my_sorted[__float_as_int(s_rest[half][l[k]].w) + rank] =
+                make_float4(P.dp[k].x, P.dp[k].y, P.dp[k].z, P.cnt[k]);
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 4; ++k) {
+            atomicAdd(&my_sorted[l[k]].x, P.dp[k].x);
+            atomicAdd(&my_sorted[l[k]].y, P.dp[k].y);
+            atomicAdd(&my_sorted[l[k]].z, P.dp[k].z);
+            if (!S.block_mode) atomicAdd(&my_sorted[l[k]].w, P.cnt[k]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (lt < h.n_verts) {
+      float4 a;
+      if (csr) {
+        a = zero;
+        const int cb = vs.x & 0xffff, ce = cb + ((vs.x >> 16) & 0x7fff);
+        int c = cb;
+        for (; c + 4 <= ce; c += 4) {
+          const float4 v0 = my_sorted[c], v1 = my_sorted[c + 1], v2 = my_sorted[c + 2],
+                       v3 = my_sorted[c + 3];
+          a.x += v0.x; a.y += v0.y; a.z += v0.z; a.w += v0.w;
+          a.x += v1.x; a.y += v1.y; a.z += v1.z; a.w += v1.w;
+          a.x += v2.x; a.y += v2.y; a.z += v2.z; a.w += v2.w;
+          a.x += v3.x; a.y += v3.y; a.z += v3.z; a.w += v3.w;
+        }
+        for (; c < ce; ++c) {
+          const float4 v = my_sorted[c];
+          a.x += v.x;
+          a.y += v.y;
+          a.z += v.z;
+          a.w += v.w;
+        }
+      } else {
+        a = my_sorted[lt];
+      }
+      S.slot_acc[vs.y] = a;
+    }
+    ok = scene_barrier(sync, ++barriers * G, timeout, &s_ok);
+    if (!ok) break;
+    // Jacobi average of the patch's vertices from every slot of each
+    if (lt < h.n_verts) s_x[half][lt] = jacobi_from(S, slot_sum(S, sb, se), val, s_y[half][lt]);
+    __syncthreads();
+    // residual at x_J: lane sums over the group's two patches in order
+    float rv[kPatchTetsPerThread], hv[kPatchTetsPerThread], dv[kPatchTetsPerThread];
+#pragma unroll
+    for (int j = 0; j < kPatchTetsPerThread; ++j) {
+      rv[j] = hv[j] = dv[j] = 0.f;
+      if (lt + j * kPatchThreads < h.n_tets) {
+        const uint2 loc = t.loc[j];
+        const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu),
+                          int((loc.x >> 16) & 0xffu), int(loc.x >> 24)};
+        residual_tet(S, f3(s_x[half][l[0]]), f3(s_x[half][l[1]]), f3(s_x[half][l[2]]),
+                     f3(s_x[half][l[3]]), idm[j], inv_det[j], t.lam[j], rv[j], hv[j], dv[j]);
+      }
+    }
+    // the second patch's terms go to LDS (the corner array is free now)
+    float* xch = reinterpret_cast<float*>(s_sorted[0]);
+    if (half == 1) {
+#pragma unroll
+      for (int j = 0; j < kPatchTetsPerThread; ++j) {
+        xch[(3 * j + 0) * kPatchThreads + lt] = rv[j];
+        xch[(3 * j + 1) * kPatchThreads + lt] = hv[j];
+        xch[(3 * j + 2) * kPatchThreads + lt] = dv[j];
+      }
+    }
+    __syncthreads();
+    float acc_r = 0.f, acc_h = 0.f, acc_d = 0.f;
+    if (half == 0) {
+#pragma unroll
+      for (int j = 0; j < kPatchTetsPerThread; ++j) {
+        acc_r += rv[j];
+        acc_h += hv[j];
+        acc_d += dv[j];
+      }
+#pragma unroll
+      for (int j = 0; j < kPatchTetsPerThread; ++j) {
+        acc_r += xch[(3 * j + 0) * kPatchThreads + lt];
+        acc_h += xch[(3 * j + 1) * kPatchThreads + lt];
+        acc_d += xch[(3 * j + 2) * kPatchThreads + lt];
+      }
+    }
+    double tot[3];
+    block_sums(acc_r, acc_h, acc_d, S.monitor != 0, tot);
+    if (tid == 0) store_partial(S, blockIdx.x, tot);
+    ok = scene_barrier(sync, ++barriers * G, timeout, &s_ok);
+  }
+  if (ok) {
+    // the update after the last iteration (k_reduce), the final blend
+    // (k_final_blend) and advance (k_advance)
+    double tot[3];
+    scene_sums_first(S, true, tot, s_red);
+    if (tid == 0) {
+      double L[6];
+      const SceneCtrl c = ctrl_update(S, s_ctrl, S.iters - 1, tot, L);
+      s_omega = float(c.omega);
+      if (writer) {
+        S.ctrl[0] = c;
+        S.ctrl[1] = c;
+        store_log_row(S, S.iters - 1, L);
+      }
+    }
+    __syncthreads();
+    if (lt < h.n_verts && vs.x < 0) {  // the vertex's owner patch writes it
+      const float4 pf = cheb_blend(S, s_y[half][lt], s_x[half][lt], s_prev[half][lt], s_omega);
+      S.pos[g] = pf;
+      S.xj[g] = s_x[half][lt];
+      advance_vertex(S, g, pf, xs0);
+    }
+#pragma unroll
+    for (int j = 0; j < kPatchTetsPerThread; ++j)
+      if (lt + j * kPatchThreads < h.n_tets) S.lam[h.tet_begin + lt + j * kPatchThreads] = t.lam[j];
+  }
+  // the last workgroup out resets the barrier counters for the next launch
+  __syncthreads();
+  if (tid == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned done =
+        __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (done == G - 1) {
+      __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 
@@ -1059,6 +1422,17 @@ hipError_t launch_final_blend(const SceneDev* scenes, const WorkItem* vmap, int 
                               int n_iter, hipStream_t st) {
   k_final_blend<<<nblocks, bs, 0, st>>>(scenes, vmap, n_iter);
   return hipGetLastError();
+}
+hipError_t launch_substep_persistent(const SceneDev* scenes, const PatchItem* items, int n_groups,
+                                    unsigned long long timeout_ticks, hipStream_t st) {
+  if (n_groups <= 0) return hipErrorInvalidValue;
+  k_substep_persistent<<<n_groups, kPersistThreads, 0, st>>>(scenes, items, n_groups,
+                                                             timeout_ticks);
+  return hipGetLastError();
+}
+hipError_t persistent_occupancy(int* blocks_per_cu) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_substep_persistent,
+                                                      kPersistThreads, 0);
 }
 hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                           hipStream_t st) {

@@ -33,6 +33,8 @@ SDF_HANG = 3
 MAX_HANG_POINTS = 4096
 PATH_KERNELS = 0
 PATH_GRAPH = 1
+PATH_PERSISTENT = 2
+PATH_AUTO = 3
 LAYOUT_FLAT = 0
 LAYOUT_PATCH = 1
 ORDER_NONE = 0

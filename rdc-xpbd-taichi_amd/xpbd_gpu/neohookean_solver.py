@@ -16,7 +16,10 @@ Execution options with no reference counterpart are keyword-only:
                    coordinate bisection; True, the default), "morton" or
                    "none" (the caller's order; False)
   device           HIP device ordinal
-  exec_path        "graph" (hipGraph per substep) or "kernels"
+  exec_path        "auto" (default: "persistent" for scenes of at most 96
+                   patches, "graph" otherwise), "graph" (hipGraph per substep),
+                   "persistent" (one launch per substep, scenes of at most 256
+                   patches) or "kernels"
   layout           "patch" (workgroup per LDS-staged patch, default) or "flat"
   log_residuals    keep r_hat / rho_hat / omega per iteration in
                    `residual_history` (one device sync per substep)
@@ -42,7 +45,8 @@ def calc_second_lame(E, nu):
 
 
 _ACCUM = {"atomic": _abi.ACCUM_ATOMIC, "csr": _abi.ACCUM_CSR}
-_PATHS = {"graph": _abi.PATH_GRAPH, "kernels": _abi.PATH_KERNELS}
+_PATHS = {"graph": _abi.PATH_GRAPH, "kernels": _abi.PATH_KERNELS,
+          "persistent": _abi.PATH_PERSISTENT, "auto": _abi.PATH_AUTO}
 _LAYOUTS = {"patch": _abi.LAYOUT_PATCH, "flat": _abi.LAYOUT_FLAT}
 _ORDERS = {"bisect": _abi.ORDER_BISECT, "morton": _abi.ORDER_MORTON, "none": _abi.ORDER_NONE,
            True: _abi.ORDER_BISECT, False: _abi.ORDER_NONE}
@@ -80,7 +84,7 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
                  accumulate_mode="csr",
                  reorder=True,
                  device=0,
-                 exec_path="graph",
+                 exec_path="auto",
                  layout="patch",
                  log_residuals=False):
         self.constraints_iter = constraints_iter

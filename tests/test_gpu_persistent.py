@@ -1,0 +1,112 @@
+"""The persistent substep kernel (exec_path="persistent": one launch per
+substep for small scenes, every iteration inside it with scene-wide barriers)
+against the phased kernels replayed from a graph: bit-identical positions,
+velocities, multipliers and per-iteration logs in deterministic mode, over
+contact, split mode with the convergence monitor, the iteration counts at the
+edges of the sequence, and the reference's demo (cow) case; in atomic mode
+against the oracle."""
+import numpy as np
+import pytest
+
+import golden_cases
+from scenes import dropped_cube, make_oracle, make_solver, resting_cube
+from test_gpu_parity import check_positions, check_residual_curve, residual_rows, run_pair
+from xpbd_gpu.mesh import box_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(scene, frames, setup=None, **opts):
+    out = []
+    for path in ("graph", "persistent"):
+        s = make_solver(scene, log_residuals=True, exec_path=path, **opts)
+        if setup is not None:
+            setup(s)
+        for _ in range(frames):
+            s.solve()
+        out.append(s)
+    return out
+
+
+def _same(a, b):
+    assert np.array_equal(a.mesh.verts.x.to_numpy(), b.mesh.verts.x.to_numpy())
+    assert np.array_equal(a.mesh.verts.v.to_numpy(), b.mesh.verts.v.to_numpy())
+    assert np.array_equal(a.mesh.cells.lambda_D.to_numpy(), b.mesh.cells.lambda_D.to_numpy())
+    assert np.array_equal(a.mesh.cells.lambda_H.to_numpy(), b.mesh.cells.lambda_H.to_numpy())
+    assert np.array_equal(residual_rows(a), residual_rows(b))
+    assert a.rho_hat == b.rho_hat
+
+
+@pytest.mark.parametrize("name,scene", [
+    ("dropped_cube16", dropped_cube()),
+    ("resting_cube16", resting_cube()),
+    ("resting_cube5", resting_cube(n=5, frames=4)),
+])
+def test_persistent_matches_graph_bitwise(name, scene):
+    print(f"[case] persistent {name}", flush=True)
+    a, b = _pair(scene, scene.frames)
+    _same(a, b)
+
+
+@pytest.mark.parametrize("iters", [1, 2, 5, 6, 40])
+def test_persistent_iteration_counts(iters):
+    print(f"[case] persistent iters {iters}", flush=True)
+    scene = resting_cube(n=6, frames=2)
+    scene.params["constraints_iter"] = iters
+    a, b = _pair(scene, 2)
+    _same(a, b)
+
+
+def test_persistent_split_mode_and_monitor():
+    print("[case] persistent split+monitor", flush=True)
+    scene = resting_cube(n=8, frames=3)
+    scene.params["block_neohookean_enable"] = False
+    scene.params["monitor_convergence"] = True
+    a, b = _pair(scene, 3)
+    _same(a, b)
+    ha = [(r["hydro_error"], r["dev_error"]) for r in a.convergence_history]
+    hb = [(r["hydro_error"], r["dev_error"]) for r in b.convergence_history]
+    assert ha == hb
+
+
+def test_persistent_reference_demo():
+    print("[case] persistent cow demo", flush=True)
+    case = golden_cases.load("cow_demo_contact")
+    sc = golden_cases.scene(case)
+
+    def setup(s):
+        s.mesh.verts.v.from_numpy(case["v0"])
+        s.rho_hat = case["rho0"]
+    a, b = _pair(sc, case["frames"], setup=setup)
+    _same(a, b)
+
+
+def test_persistent_hang_and_cube_colliders():
+    from xpbd_gpu.scenes import Scene, base_params, cube, hang, plane
+    print("[case] persistent colliders", flush=True)
+    m = box_grid(4, origin=(0.0, 0.3, 0.0))
+    sdfs = [cube((0.5, 0.1, 0.5), (2.0, 0.4, 2.0), fixed=False),
+            hang([[0.0, 1.3, 0.0], [1.0, 1.3, 1.0]]), plane(-1.0)]
+    sc = Scene("cube_hang", m, sdfs, base_params(young_modulus=2e5, poisson_ratio=0.4,
+                                                 substeps_num=4, constraints_iter=20),
+               total_mass=1000.0, frames=5)
+    a, b = _pair(sc, 5)
+    _same(a, b)
+
+
+def test_persistent_atomic_mode_matches_oracle():
+    print("[case] persistent atomic", flush=True)
+    scene = resting_cube(frames=5)
+    solver, orc = run_pair(scene, accumulate_mode="atomic", exec_path="persistent")
+    check_positions(scene, solver, orc)
+    check_residual_curve(solver, orc)
+
+
+def test_large_scene_falls_back_to_graph():
+    """Above the persistent kernel's size limit the solver takes the graph path
+    (same bits)."""
+    from scenes import jittered_box
+    print("[case] persistent fallback", flush=True)
+    scene = jittered_box(n=24, frames=1, iters=10)
+    a, b = _pair(scene, 1)
+    _same(a, b)
