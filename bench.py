@@ -170,7 +170,9 @@ def summarize_small_stats(folder: str) -> str:
         "# rocprofv3 kernel statistics of the small single-scene configurations: the 16^3 cube",
         "# (24 576 tets, 5 substeps x 30 iterations per frame) falling and resting on the plane,",
         "# and the 32^3 twisted grid (196 608 tets, 200 iterations per frame); each 1 untimed +",
-        "# 10 timed frames.  Kernels of both sizes are pooled per name.",
+        "# 10 timed frames on the default path (auto: the cube's substeps run as one",
+        "# k_substep_persistent launch each, the grid as phased kernels) and again on the graph",
+        "# path (phased kernels for both).  Kernels of all runs are pooled per name.",
         "# Made by: " + SMALL_STATS_COMMAND])
 
 
