@@ -3,12 +3,15 @@
 Each request asks to advance one scene by a number of frames.  Scenes are kept
 resident in HBM: created on first use and admitted only while the sum of their
 device footprints fits the budget; idle scenes are evicted least recently used
-first, their state parked in host memory and restored when they return.  At
-every substep boundary the loop admits the requests that have arrived and whose
-scene is free and resident, steps all active scenes by one substep with a single
-batched launch sequence (`SceneBatch`), and retires the requests that have
-completed.  Requests that arrive while others run join the
-batch at the next boundary, so their work overlaps on the device.
+first, their state parked in host memory and restored when they return.  The
+loop admits the requests that have arrived and whose scene is free and
+resident, then enqueues substeps of all active scenes (one batched launch
+sequence per substep, `SceneBatch`, replayed from a captured graph while the
+membership holds) up to the next request completion -- at most one frame, so
+that new arrivals wait at most that long for admission -- and only then waits
+for the device and retires the completed requests.  Requests that arrive while
+others run join the batch at the next such boundary, so their work overlaps on
+the device.
 
 `run_sequential` serves the same trace one request at a time with plain
 per-scene solves, the baseline the batched loop is compared with.
@@ -46,7 +49,13 @@ class Done:
 
     @property
     def latency(self) -> float:
+        """Arrival to completion."""
         return self.finish - self.arrival
+
+    @property
+    def service(self) -> float:
+        """Admission (start of its first substep) to completion."""
+        return self.finish - self.start
 
 
 def served_scenes(n_scenes: int = 64, seed: int = 0, median_tets: float = 2e4,
@@ -99,11 +108,13 @@ class Server:
     """Resident scenes, admission by device-memory budget, continuous batching."""
 
     def __init__(self, scenes: list, hbm_budget: float = 200e9, batch_opts=None,
-                 solver_opts=None):
+                 solver_opts=None, max_run: int | None = None):
         self.specs = list(scenes)
         self.budget = float(hbm_budget)
-        self.batch_opts = dict(graph=False)
+        self.batch_opts = dict(graph=True)
         self.batch_opts.update(batch_opts or {})
+        # substeps enqueued between two waits for the device (default: a frame)
+        self.max_run = max_run
         self.solver_opts = dict(solver_opts or {})
         self.resident = collections.OrderedDict()  # scene -> solver, least recently used first
         self.bytes = {}
@@ -112,9 +123,28 @@ class Server:
         self.rebuilds = 0
         self.substeps_run = 0
         self.evictions = 0
-        # host seconds spent building batches and stepping them (launch + wait)
+        # host seconds: re-targeting batches, enqueueing substeps (launches or
+        # graph replays, graph capture included), waiting for the device
         self.build_s = 0.0
-        self.step_s = 0.0
+        self.launch_s = 0.0
+        self.wait_s = 0.0
+        self.syncs = 0
+
+    @property
+    def step_s(self) -> float:
+        """Host seconds spent stepping batches (enqueue + wait)."""
+        return self.launch_s + self.wait_s
+
+    def reset_counters(self):
+        self.rebuilds = self.substeps_run = self.syncs = 0
+        self.build_s = self.launch_s = self.wait_s = 0.0
+
+    def close(self):
+        """Frees every resident scene (parked states stay in host memory)."""
+        for s in self.resident.values():
+            s.close()
+        self.resident.clear()
+        self.bytes.clear()
 
     @property
     def used_bytes(self) -> int:
@@ -200,15 +230,21 @@ class Server:
                 key = k
                 self.rebuilds += 1
                 self.build_s += time.perf_counter() - tb
+            # every active request runs at least `run` more substeps
+            run = min(e[1] for e in active.values())
+            run = min(run, self.max_run or min(self.specs[j].params["substeps_num"] for j in k))
             ts = time.perf_counter()
-            batch.substep()
+            batch.substep(run)
+            tw = time.perf_counter()
             batch.synchronize()
-            self.step_s += time.perf_counter() - ts
-            self.substeps_run += 1
+            self.launch_s += tw - ts
+            self.wait_s += time.perf_counter() - tw
+            self.substeps_run += run
+            self.syncs += 1
             now = time.perf_counter() - t0
             for j in k:
                 entry = active[j]
-                entry[1] -= 1
+                entry[1] -= run
                 if entry[1] == 0:
                     req = entry[0]
                     s = self.resident[j]
@@ -244,12 +280,15 @@ def run_sequential(scenes: list, requests: list, solver_opts=None, solvers=None)
 
 def summarize(done: list, wall: float | None = None) -> dict:
     lat = np.array([d.latency for d in done]) if done else np.zeros(1)
+    svc = np.array([d.service for d in done]) if done else np.zeros(1)
     work = sum(d.tet_iterations for d in done)
     span = wall if wall is not None else (max(d.finish for d in done) if done else 0.0)
     return {
         "requests": len(done),
         "p50_ms": float(np.percentile(lat, 50) * 1e3),
         "p99_ms": float(np.percentile(lat, 99) * 1e3),
+        "service_p50_ms": float(np.percentile(svc, 50) * 1e3),
+        "service_p99_ms": float(np.percentile(svc, 99) * 1e3),
         "tet_iterations": int(work),
         "makespan_s": float(span),
         "tet_iterations_per_s": float(work / span) if span > 0 else 0.0,

@@ -20,9 +20,10 @@ def _alone_after(scene, frames):
     return s.mesh.verts.x.to_numpy()
 
 
-@pytest.mark.parametrize("budget_scenes", [None, 2])
-def test_served_results_match_alone(budget_scenes):
-    _print_case("served_alone_vs_mix", budget_scenes=budget_scenes)
+@pytest.mark.parametrize("budget_scenes,graph,max_run", [(None, True, None), (2, True, None),
+                                                         (None, False, None), (2, True, 1)])
+def test_served_results_match_alone(budget_scenes, graph, max_run):
+    _print_case("served_alone_vs_mix", budget_scenes=budget_scenes, graph=graph, max_run=max_run)
     scenes = served_scenes(n_scenes=5, seed=1, median_tets=2e3, tet_range=(3e2, 6e3), iters=10)
     reqs = poisson_trace(len(scenes), rate=60.0, duration=0.5, seed=2)
     assert len(reqs) >= 10
@@ -34,10 +35,14 @@ def test_served_results_match_alone(budget_scenes):
         sizes = sorted(srv0.bytes.values())
         budget = sizes[-1] + sizes[-2] + 1  # any two scenes fit, never all five
         del srv0
-    srv = Server(scenes, hbm_budget=budget)
+    srv = Server(scenes, hbm_budget=budget, batch_opts=dict(graph=graph), max_run=max_run)
     done = srv.run(reqs)
     assert sorted(d.rid for d in done) == sorted(r.rid for r in reqs)
     assert srv.used_bytes <= budget
+    # one wait for the device per run of substeps, never more than one per substep
+    assert srv.syncs <= srv.substeps_run
+    if max_run == 1:
+        assert srv.syncs == srv.substeps_run
     if budget_scenes is not None:
         assert srv.evictions > 0
     frames = {}
@@ -61,6 +66,11 @@ def test_requests_overlap_and_latency_accounting():
     assert done[1].start == done[0].start
     assert done[2].start >= done[0].finish
     assert all(d.finish >= d.start >= d.arrival for d in done.values())
+    assert all(d.latency >= d.service > 0 for d in done.values())
+    # request 1 (one frame) ends the first run: 5 substeps, then 0 runs 5 more
+    # alone, then 2; three waits for the device in all
+    assert srv.substeps_run == 15 and srv.syncs == 3
+    srv.close()
 
 
 def test_sequential_baseline_serves_trace():
