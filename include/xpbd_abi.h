@@ -118,7 +118,7 @@ typedef struct xpbd_solver xpbd_solver;
 
 /* Version of this ABI and the sizes of its structs as compiled, so that a
  * binding can check its own struct layout. */
-#define XPBD_ABI_VERSION 4
+#define XPBD_ABI_VERSION 5
 int xpbd_abi_info(int32_t* version, size_t* desc_size, size_t* sdf_size);
 
 /* Fills *d with the reference defaults (everything but the mesh). */
@@ -263,6 +263,14 @@ typedef struct xpbd_batch_opts {
   int32_t vertex_block;   /* threads per vertex workgroup, 64..1024, multiple of 64 */
   int32_t grid_pct;       /* persistent patch grid, percent of full occupancy, 1..100 */
   int32_t xcd_split;      /* 1: every XCD walks a contiguous share of the work (default) */
+  int32_t persistent;     /* the persistent substep (one launch per substep, each member's
+                           * workgroups meeting at that member's own barriers):
+                           * 0 never; 1 (default) when every member could take it alone,
+                           * none spans more than 96 patches and the members hold at most
+                           * 256 patches in all (one workgroup per 2 patches); 2 whenever
+                           * every member could take it alone and they fit, else the
+                           * substep fails with XPBD_EINVAL.  Other batches take the
+                           * phased kernels (graph or launches as above). */
 } xpbd_batch_opts;
 
 int xpbd_batch_opts_init(xpbd_batch_opts* o);
@@ -281,7 +289,8 @@ int xpbd_batch_set_members(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_
  * GPU, that the library holds until the process ends; the call makes no
  * device allocation or release.  Limits: at most XPBD_NOALLOC_MAX_SCENES
  * members and XPBD_NOALLOC_LIST_BYTES of work lists (32 B per patch of 512
- * tets, 8 B per vertex block and per persistent workgroup); larger member
+ * tets, 8 B per vertex block and per persistent workgroup, 16 B per
+ * workgroup of the persistent substep); larger member
  * lists are refused with XPBD_EINVAL and leave the batch as it was.  The
  * batch must not replay graphs (opts.graph == 0: capturing a new one would
  * allocate).  The array holds one batch's lists per GPU: a later call for
@@ -294,7 +303,12 @@ int xpbd_batch_set_members_noalloc(xpbd_batch* b, xpbd_solver* const* scenes, in
 int xpbd_batch_substep(xpbd_batch* b);
 /* n_frames frames of every member (solve(), neohookean_solver.py:562-587) */
 int xpbd_batch_frame(xpbd_batch* b, int32_t n_frames);
+/* Waits for the batch's queued work; XPBD_EHIP when a persistent substep's
+ * barrier gave up since the last call (its results are invalid). */
 int xpbd_batch_synchronize(xpbd_batch* b);
+/* The path the batch's next substeps take with its current members:
+ * XPBD_PATH_PERSISTENT, XPBD_PATH_GRAPH or XPBD_PATH_KERNELS. */
+int xpbd_batch_path(xpbd_batch* b, int32_t* path);
 int xpbd_batch_get_stream(xpbd_batch* b, void** stream);
 int xpbd_batch_info(xpbd_batch* b, int32_t* n_scenes, int64_t* n_verts, int64_t* n_tets,
                     int32_t* n_patches);

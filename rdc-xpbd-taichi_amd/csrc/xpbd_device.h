@@ -164,6 +164,16 @@ constexpr int kPartialGroup = 2;
 // residual group of kPartialGroup patches (one patch per half).
 constexpr int kPersistThreads = kPatchThreads * kPartialGroup;
 
+// One workgroup of the batched persistent substep: the member scene, its
+// residual group in that scene, and where the scene's patch items start in
+// the batch's concatenated item list.
+struct PersistGroup {
+  int scene;
+  int group;
+  int item0;
+  int pad;
+};
+
 // Reciprocal and square root on the native gfx950 instructions (v_rcp_f32,
 // v_sqrt_f32: about 1 ulp) instead of the correctly rounded sequences (a
 // dozen instructions each), where the result is not amplified (rest-shape

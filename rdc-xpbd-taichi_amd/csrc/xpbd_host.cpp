@@ -33,6 +33,7 @@ using xpbd::SceneDev;
 using xpbd::SdfDev;
 using xpbd::WorkItem;
 using xpbd::PatchItem;
+using xpbd::PersistGroup;
 using xpbd::kPatchTets;
 using xpbd::kPatchVerts;
 
@@ -615,13 +616,17 @@ int build_scene(xpbd_solver* s) {
 }
 
 // Uploads the scene table if (dt, damping) or any parameter changed.
+// Brings the device scene table up to date with the parameters of the next
+// substep.  The store is a launch on the solver's stream, so substeps already
+// queued keep the parameters they were queued with and the host does not wait:
+// a frame whose last time chunk is shorter (frame_dt not a multiple of dt)
+// changes sub_dt twice per frame without a round trip.
 int sync_table(xpbd_solver* s, double dt, double damping) {
   SceneDev S = s->host_scene;
   fill_scene_params(s, S, dt, damping);
   if (s->table_valid && std::memcmp(&S, &s->host_scene, sizeof(S)) == 0) return XPBD_OK;
   s->host_scene = S;
-  HIP_TRY(hipStreamSynchronize(s->stream));
-  HIP_TRY(hipMemcpy(s->table.p, &s->host_scene, sizeof(SceneDev), hipMemcpyHostToDevice));
+  HIP_TRY(xpbd::launch_store_scene(s->table.p, s->host_scene, s->stream));
   s->table_valid = true;
   return XPBD_OK;
 }
@@ -966,6 +971,13 @@ constexpr int kPersistMaxGroups = 128;                             // 256 patche
 // release and acquire per workgroup), and two per iteration beat the phased
 // launches up to about this size.
 constexpr int kAutoPersistGroups = 48;
+// A batch takes the persistent substep automatically up to this many
+// workgroups in all.  Each workgroup's agent-scope release and acquire is an
+// L2 write-back or invalidate on its XCD, and with more workgroups these
+// queue up: for members of about 20 groups an iteration costs 11.5 us at 19
+// groups in all, 13.5 us at 80, 17.3 us at 160 and 20.4 us at 222, against
+// 15.6-16.8 us for the phased launches.
+constexpr int kAutoBatchGroups = 128;
 
 // Whether the persistent substep kernel can run this scene: the fused
 // sequence with at least one iteration, and few enough groups to be resident
@@ -1759,6 +1771,14 @@ struct xpbd_batch {
   PatchItem* items_d = nullptr;
   int2* rp_d = nullptr;
   int2* rr_d = nullptr;
+  // persistent substep: one workgroup per residual group of every member, the
+  // launch's error word, and whether it has run since the word was last read
+  DevBuf<PersistGroup> pgroups;
+  PersistGroup* pgroups_d = nullptr;
+  int n_pgroups = 0;
+  DevBuf<unsigned> perr;
+  int persist_capacity = 0;
+  bool persist_pending = false;
 
   ~xpbd_batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -1862,10 +1882,13 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes, bool no
   const xpbd_batch_opts& o = b->opts;
   std::vector<WorkItem> vm;
   std::vector<PatchItem> items;
+  std::vector<PersistGroup> pg;
   int64_t n_verts = 0, n_tets = 0;
   for (int i = 0; i < n_scenes; ++i) {
     xpbd_solver* s = scenes[i];
     for (int v = 0; v < s->nv; v += o.vertex_block) vm.push_back({i, v});
+    if (s->layout == XPBD_LAYOUT_PATCH)
+      for (int g = 0; g < s->host_scene.n_partials; ++g) pg.push_back({i, g, int(items.size()), 0});
     for (PatchItem it : s->pitems_host) {
       it.scene = i;
       items.push_back(it);
@@ -1891,7 +1914,8 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes, bool no
   if (noalloc) {
     auto a256 = [](size_t x) { return (x + 255) / 256 * 256; };
     const size_t bytes = a256(sizeof(WorkItem) * vmo.size()) + a256(sizeof(PatchItem) * items.size()) +
-                         a256(sizeof(int2) * rp.size()) + a256(sizeof(int2) * rr.size());
+                         a256(sizeof(int2) * rp.size()) + a256(sizeof(int2) * rr.size()) +
+                         a256(sizeof(PersistGroup) * pg.size());
     if (n_scenes > XPBD_NOALLOC_MAX_SCENES)
       return set_error(XPBD_EINVAL, "%d members: a batch re-targeted without allocation holds "
                        "at most %d", n_scenes, XPBD_NOALLOC_MAX_SCENES);
@@ -1916,6 +1940,7 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes, bool no
     TRY(place(b->items_d, items));
     TRY(place(b->rp_d, rp));
     TRY(place(b->rr_d, rr));
+    TRY(place(b->pgroups_d, pg));
     b->table_d = static_cast<SceneDev*>(table);
     b->noalloc = true;
   } else {
@@ -1924,15 +1949,18 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes, bool no
     TRY(b->pitems.upload_reuse(items.data(), items.size()));
     TRY(b->prange_proj.upload_reuse(rp.data(), rp.size()));
     TRY(b->prange_res.upload_reuse(rr.data(), rr.size()));
+    TRY(b->pgroups.upload_reuse(pg.data(), pg.size()));
     if (b->table.cap < size_t(n_scenes)) TRY(b->table.alloc(n_scenes));
     b->table.n = size_t(n_scenes);
     b->vmap_d = b->vmap.p;
     b->items_d = b->pitems.p;
     b->rp_d = b->prange_proj.p;
     b->rr_d = b->prange_res.p;
+    b->pgroups_d = b->pgroups.p;
     b->table_d = b->table.p;
     b->noalloc = false;
   }
+  b->n_pgroups = int(pg.size());
   b->scenes.assign(scenes, scenes + n_scenes);
   b->n_verts = n_verts;
   b->n_tets = n_tets;
@@ -1989,6 +2017,21 @@ void batch_count_substep(xpbd_batch* b) {
   }
 }
 
+// Whether the batch's next substeps take the persistent substep kernel: every
+// member could take it alone, and all their workgroups fit on the device at
+// once.  opts.persistent 1 (automatic) also asks, as XPBD_PATH_AUTO does for one
+// scene, that no member's barrier spans more than kAutoPersistGroups
+// workgroups, and that the launch stays within kAutoBatchGroups.
+bool batch_persistent(const xpbd_batch* b) {
+  if (b->opts.persistent == 0 || b->n_pgroups < 1 || b->n_pgroups > b->persist_capacity)
+    return false;
+  for (const xpbd_solver* s : b->scenes) {
+    if (!persistent_ok(s)) return false;
+    if (b->opts.persistent == 1 && s->host_scene.n_partials > kAutoPersistGroups) return false;
+  }
+  return b->opts.persistent == 2 || b->n_pgroups <= kAutoBatchGroups;
+}
+
 bool batch_at_frame_start(const xpbd_batch* b) {
   for (const xpbd_solver* s : b->scenes)
     if (s->sub_pos != 0) return false;
@@ -2003,6 +2046,20 @@ int batch_run(xpbd_batch* b, int n_substeps) {
                                     "no-allocation array of device %d: re-target it", b->device);
   }
   TRY(batch_sync_table(b));
+  if (batch_persistent(b)) {
+    for (int k = 0; k < n_substeps; ++k) {
+      HIP_TRY(xpbd::launch_batch_substep_persistent(b->table_d, b->items_d, b->pgroups_d,
+                                                    b->n_pgroups, b->perr.p, kPersistTimeoutTicks,
+                                                    b->stream));
+      b->persist_pending = true;
+      batch_count_substep(b);
+    }
+    return XPBD_OK;
+  }
+  if (b->opts.persistent == 2)
+    return set_error(XPBD_EINVAL, "the persistent substep does not apply to this batch (patch "
+                                  "layout, fused sequence, at most %d groups per member and %d "
+                                  "in all)", kPersistMaxGroups, b->persist_capacity);
   if (!b->opts.graph) {
     for (int k = 0; k < n_substeps; ++k) {
       TRY(batch_enqueue_substep(b));
@@ -2055,6 +2112,7 @@ int xpbd_batch_opts_init(xpbd_batch_opts* o) {
   o->vertex_block = 256;
   o->grid_pct = 100;
   o->xcd_split = 1;
+  o->persistent = 1;
   return XPBD_OK;
 }
 
@@ -2071,11 +2129,20 @@ int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_b
   if (o.vertex_block < 64 || o.vertex_block > 1024 || o.vertex_block % 64)
     return set_error(XPBD_EINVAL, "vertex_block must be a multiple of 64 in [64, 1024]");
   if (o.grid_pct < 1 || o.grid_pct > 100) return set_error(XPBD_EINVAL, "grid_pct in [1, 100]");
+  if (o.persistent < 0 || o.persistent > 2) return set_error(XPBD_EINVAL, "persistent in 0..2");
   if (!scenes[0]) return set_error(XPBD_EINVAL, "NULL scene");
   b->device = scenes[0]->device;
   HIP_TRY(hipSetDevice(b->device));
   TRY(batch_check_members(scenes, n_scenes, b->device));
   HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  {
+    int occ = 0, n_cu = 0;
+    HIP_TRY(xpbd::persistent_occupancy(&occ));
+    HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
+    b->persist_capacity = n_cu * occ;
+  }
+  TRY(b->perr.alloc(1));
+  TRY(b->perr.zero());
   TRY(batch_build(b.get(), scenes, n_scenes, false));
   *out = b.release();
   return XPBD_OK;
@@ -2141,6 +2208,23 @@ int xpbd_batch_synchronize(xpbd_batch* b) {
   if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
   HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipStreamSynchronize(b->stream));
+  if (!b->persist_pending) return XPBD_OK;
+  b->persist_pending = false;
+  unsigned err = 0;
+  HIP_TRY(hipMemcpy(&err, b->perr.p, sizeof(err), hipMemcpyDeviceToHost));
+  if (err) {
+    TRY(b->perr.zero());
+    return set_error(XPBD_EHIP, "persistent batch substep: a scene barrier timed out (its "
+                                "workgroups were not all resident); the results are invalid");
+  }
+  return XPBD_OK;
+}
+
+int xpbd_batch_path(xpbd_batch* b, int32_t* path) {
+  if (!b || !path) return set_error(XPBD_EINVAL, "NULL argument");
+  TRY(batch_sync_table(b));
+  *path = batch_persistent(b) ? XPBD_PATH_PERSISTENT
+                              : (b->opts.graph ? XPBD_PATH_GRAPH : XPBD_PATH_KERNELS);
   return XPBD_OK;
 }
 

@@ -38,6 +38,15 @@ hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblo
 // a scene barrier gives up after timeout_ticks of the 100 MHz wall clock.
 hipError_t launch_substep_persistent(const SceneDev* scenes, const PatchItem* items, int n_groups,
                                     unsigned long long timeout_ticks, hipStream_t st);
+// One substep of every member of a batch in one launch: workgroup b does
+// groups[b] (n_groups in all, all resident); a barrier that gives up sets *err.
+hipError_t launch_batch_substep_persistent(const SceneDev* scenes, const PatchItem* items,
+                                          const PersistGroup* groups, int n_groups,
+                                          unsigned* err, unsigned long long timeout_ticks,
+                                          hipStream_t st);
+// Stores *dst = src in stream order (no host wait).
+hipError_t launch_store_scene(SceneDev* dst, const SceneDev& src, hipStream_t st);
+// Workgroups of either persistent substep kernel resident per CU.
 hipError_t persistent_occupancy(int* blocks_per_cu);
 hipError_t launch_scatter_xyz(float4* dst, const float* src, const int* old_to_new, int n,
                               int keep_w, hipStream_t st);

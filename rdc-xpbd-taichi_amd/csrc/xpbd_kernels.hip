@@ -950,19 +950,20 @@ __global__ void k_advance(const SceneDev* __restrict__ scenes, const WorkItem* _
 
 // ---- persistent substep (small scenes) -------------------------------------------
 
-// A scene-wide barrier over the persistent substep's workgroups.  The
-// workgroup barrier orders every thread's global writes before thread 0
+// A scene-wide barrier over the persistent substep's workgroups of one scene.
+// The workgroup barrier orders every thread's global writes before thread 0
 // (work-group scope), which releases them at agent scope, counts the
-// workgroup in and waits, without blocking forever, until `target` arrivals
-// are counted; it then acquires at agent scope, and the closing workgroup
-// barrier passes that on to the other threads.  (One release and one acquire
-// per workgroup: each is an L2 write-back or invalidate on this GPU.)  The
-// wait ends early when another workgroup has set the error word, and after
-// `timeout` ticks of the 100 MHz wall clock it sets the error word itself
-// (the workgroups are not all resident).  Returns false on either: the
-// caller then leaves its loops.
-__device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, unsigned target,
-                                              unsigned long long timeout, int* s_ok) {
+// workgroup in on the scene's arrival word and waits, without blocking
+// forever, until `target` arrivals are counted; it then acquires at agent
+// scope, and the closing workgroup barrier passes that on to the other
+// threads.  (One release and one acquire per workgroup: each is an L2
+// write-back or invalidate on this GPU.)  The wait ends early when any
+// workgroup has set the error word, and after `timeout` ticks of the 100 MHz
+// wall clock it sets the error word itself (the workgroups are not all
+// resident).  Returns false on either: the caller then leaves its loops.
+__device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, gptr<unsigned> err,
+                                              unsigned target, unsigned long long timeout,
+                                              int* s_ok) {
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
@@ -970,12 +971,12 @@ __device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, unsigned targ
     __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long t0 = wall_clock64();
     while (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(&sync[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+      if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
         ok = 0;
         break;
       }
       if (wall_clock64() - t0 > timeout) {
-        __hip_atomic_fetch_or(&sync[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ok = 0;
         break;
       }
@@ -1045,14 +1046,34 @@ __device__ __forceinline__ void scene_sums_first(const SceneDev& S, bool all3, d
 // partials.  Every sum is taken in the phased kernels' order, so the results
 // are bit-identical to them in deterministic mode.  Requires the fused
 // sequence (patch layout, Chebyshev on, no isolated vertices), at least one
-// iteration, and all n_groups workgroups resident at once.
+// iteration, and all of the launch's workgroups resident at once.
+//
+// kBatch = false: one scene (scenes[0]), workgroup g = residual group g, and
+// the scene's own error word (sync[2]).  kBatch = true: the members of a
+// batch side by side, workgroup b doing groups[b]; each scene's workgroups
+// meet at that scene's barrier words only, and one error word serves the
+// launch.  A member's results are the same bits as alone.
+template <bool kBatch>
 __global__ void __launch_bounds__(kPersistThreads)
     k_substep_persistent(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
-                         int n_groups, unsigned long long timeout) {
-  const SceneDev& S = scenes[0];
+                         const PersistGroup* __restrict__ groups, unsigned* err_word,
+                         unsigned long long timeout) {
+  int scene = 0, group = blockIdx.x;
+  if constexpr (kBatch) {
+    const PersistGroup d = groups[blockIdx.x];
+    scene = d.scene;
+    group = d.group;
+    items += d.item0;
+  }
+  const SceneDev& S = scenes[scene];
+  gptr<unsigned> err;
+  if constexpr (kBatch)
+    err = (gptr<unsigned>)err_word;
+  else
+    err = S.sync + 2;
   const int tid = threadIdx.x;
   const int half = tid / kPatchThreads, lt = tid % kPatchThreads;
-  const int p = blockIdx.x * kPartialGroup + half;
+  const int p = group * kPartialGroup + half;
   const bool active = p < S.n_patches;
   const bool csr = S.csr != 0;
   const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1111,10 +1132,10 @@ __global__ void __launch_bounds__(kPersistThreads)
     rest_shape(f3(s_rest[half][l[0]]), f3(s_rest[half][l[1]]), f3(s_rest[half][l[2]]),
                f3(s_rest[half][l[3]]), idm[j], inv_det[j]);
   }
-  const unsigned G = unsigned(n_groups);
+  const unsigned G = unsigned(S.n_partials);
   unsigned barriers = 0;
   bool ok = true;
-  const bool writer = blockIdx.x == 0;
+  const bool writer = group == 0;
   for (int it = 0; it < S.iters && ok; ++it) {
     if (it > 0) {
       // rho/omega update after iteration it - 1 from every group's partial
@@ -1196,7 +1217,7 @@ __global__ void __launch_bounds__(kPersistThreads)
       }
       S.slot_acc[vs.y] = a;
     }
-    ok = scene_barrier(sync, ++barriers * G, timeout, &s_ok);
+    ok = scene_barrier(sync, err, ++barriers * G, timeout, &s_ok);
     if (!ok) break;
     // Jacobi average of the patch's vertices from every slot of each
     if (lt < h.n_verts) s_x[half][lt] = jacobi_from(S, slot_sum(S, sb, se), val, s_y[half][lt]);
@@ -1242,8 +1263,8 @@ __global__ void __launch_bounds__(kPersistThreads)
     }
     double tot[3];
     block_sums(acc_r, acc_h, acc_d, S.monitor != 0, tot);
-    if (tid == 0) store_partial(S, blockIdx.x, tot);
-    ok = scene_barrier(sync, ++barriers * G, timeout, &s_ok);
+    if (tid == 0) store_partial(S, group, tot);
+    ok = scene_barrier(sync, err, ++barriers * G, timeout, &s_ok);
   }
   if (ok) {
     // the update after the last iteration (k_reduce), the final blend
@@ -1282,6 +1303,13 @@ __global__ void __launch_bounds__(kPersistThreads)
       __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
+}
+
+// Writes one scene table entry from the launch's arguments.  Being a launch
+// on the scene's stream, it takes effect after the substeps queued before it
+// and before those queued after, with no wait on the host.
+__global__ void k_store_scene(SceneDev* __restrict__ dst, const SceneDev src) {
+  if (threadIdx.x == 0) *dst = src;
 }
 
 // Copies between a caller-order (V,3) float array and an internal float4
@@ -1426,13 +1454,32 @@ hipError_t launch_final_blend(const SceneDev* scenes, const WorkItem* vmap, int 
 hipError_t launch_substep_persistent(const SceneDev* scenes, const PatchItem* items, int n_groups,
                                     unsigned long long timeout_ticks, hipStream_t st) {
   if (n_groups <= 0) return hipErrorInvalidValue;
-  k_substep_persistent<<<n_groups, kPersistThreads, 0, st>>>(scenes, items, n_groups,
-                                                             timeout_ticks);
+  k_substep_persistent<false><<<n_groups, kPersistThreads, 0, st>>>(scenes, items, nullptr,
+                                                                    nullptr, timeout_ticks);
+  return hipGetLastError();
+}
+hipError_t launch_batch_substep_persistent(const SceneDev* scenes, const PatchItem* items,
+                                          const PersistGroup* groups, int n_groups,
+                                          unsigned* err, unsigned long long timeout_ticks,
+                                          hipStream_t st) {
+  if (n_groups <= 0 || !groups || !err) return hipErrorInvalidValue;
+  k_substep_persistent<true><<<n_groups, kPersistThreads, 0, st>>>(scenes, items, groups, err,
+                                                                   timeout_ticks);
   return hipGetLastError();
 }
 hipError_t persistent_occupancy(int* blocks_per_cu) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_substep_persistent,
-                                                      kPersistThreads, 0);
+  int a = 0, b = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_substep_persistent<false>,
+                                                              kPersistThreads, 0);
+  if (e != hipSuccess) return e;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_substep_persistent<true>,
+                                                   kPersistThreads, 0);
+  *blocks_per_cu = a < b ? a : b;
+  return e;
+}
+hipError_t launch_store_scene(SceneDev* dst, const SceneDev& src, hipStream_t st) {
+  k_store_scene<<<1, 64, 0, st>>>(dst, src);
+  return hipGetLastError();
 }
 hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblocks, int bs,
                           hipStream_t st) {

@@ -22,7 +22,7 @@ XPBD_ENOMEM = -3
 XPBD_ESTATE = -4
 XPBD_ENUMERIC = -5
 XPBD_MAX_SDF = 8
-XPBD_ABI_VERSION = 4
+XPBD_ABI_VERSION = 5
 
 ACCUM_ATOMIC = 0
 ACCUM_CSR = 1
@@ -59,6 +59,7 @@ EXPORTED = (
     "xpbd_batch_destroy", "xpbd_batch_set_members", "xpbd_batch_set_members_noalloc",
     "xpbd_batch_substep", "xpbd_batch_frame",
     "xpbd_batch_synchronize", "xpbd_batch_get_stream", "xpbd_batch_info", "xpbd_batch_grid",
+    "xpbd_batch_path",
 )
 
 
@@ -116,6 +117,7 @@ class XpbdBatchOpts(ctypes.Structure):
         ("vertex_block", ctypes.c_int32),
         ("grid_pct", ctypes.c_int32),
         ("xcd_split", ctypes.c_int32),
+        ("persistent", ctypes.c_int32),
     ]
 
 
@@ -214,6 +216,7 @@ def load() -> ctypes.CDLL:
                             ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32)],
         "xpbd_batch_grid": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                             ctypes.POINTER(ctypes.c_int32)],
+        "xpbd_batch_path": [vp, ctypes.POINTER(ctypes.c_int32)],
         "xpbd_patch_stats": [fp, ctypes.POINTER(ctypes.c_int32), i32, i32,
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],

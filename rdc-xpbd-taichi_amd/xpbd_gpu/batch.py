@@ -3,6 +3,9 @@
 A `SceneBatch` concatenates the patches and vertex blocks of its member
 `XPBDNeoHookeanSolver` scenes into one work list, so each kernel of a substep
 (and one replayed hipGraph per substep or per frame) covers every scene.
+Batches of few small scenes (each at most 96 patches, 256 in all) take the
+persistent substep instead: one launch per substep, in which each member's
+workgroups meet at that member's own barriers (option `persistent`).
 Scenes keep their own buffers, materials, SDFs and rho/omega state; the
 results of a scene are bit-identical alone and in any batch.
 
@@ -19,7 +22,9 @@ import time
 from . import _abi
 
 DEFAULT_OPTS = dict(graph=True, graph_substeps=1, vertex_block=256, grid_pct=100,
-                    xcd_split=True)
+                    xcd_split=True, persistent="auto")
+
+_PERSISTENT = {"never": 0, False: 0, "auto": 1, "always": 2, True: 2}
 
 
 def _opts_struct(opts: dict) -> _abi.XpbdBatchOpts:
@@ -30,6 +35,10 @@ def _opts_struct(opts: dict) -> _abi.XpbdBatchOpts:
     o.vertex_block = int(opts.get("vertex_block", 256))
     o.grid_pct = int(opts.get("grid_pct", 100))
     o.xcd_split = int(bool(opts.get("xcd_split", True)))
+    per = opts.get("persistent", "auto")
+    if per not in _PERSISTENT:
+        raise ValueError(f"persistent must be one of 'never', 'auto', 'always' (got {per!r})")
+    o.persistent = _PERSISTENT[per]
     return o
 
 
@@ -76,6 +85,15 @@ class SceneBatch:
         _abi.check(self._lib.xpbd_batch_grid(self._h, ctypes.byref(gp), ctypes.byref(gr),
                                              ctypes.byref(ms)))
         return {"project": gp.value, "residual": gr.value, "max_scenes_per_range": ms.value}
+
+    @property
+    def path(self) -> str:
+        """The path the next substeps take: "persistent" (one launch per
+        substep), "graph" or "kernels" (xpbd_batch_path)."""
+        v = ctypes.c_int32()
+        _abi.check(self._lib.xpbd_batch_path(self._h, ctypes.byref(v)))
+        return {_abi.PATH_PERSISTENT: "persistent", _abi.PATH_GRAPH: "graph",
+                _abi.PATH_KERNELS: "kernels"}[v.value]
 
     def set_members(self, solvers):
         """Re-targets the batch at another list of scenes (xpbd_batch_set_members):

@@ -5,9 +5,10 @@ resident in HBM: created on first use and admitted only while the sum of their
 device footprints fits the budget; idle scenes are evicted least recently used
 first, their state parked in host memory and restored when they return.  The
 loop admits the requests that have arrived and whose scene is free and
-resident, then enqueues substeps of all active scenes (one batched launch
-sequence per substep, `SceneBatch`, replayed from a captured graph while the
-membership holds) up to the next request completion -- at most one frame, so
+resident, then enqueues substeps of all active scenes (`SceneBatch`: for few
+small scenes one persistent launch per substep, otherwise one batched launch
+sequence per substep replayed from a captured graph while the membership
+holds) up to the next request completion -- at most one frame, so
 that new arrivals wait at most that long for admission -- and only then waits
 for the device and retires the completed requests.  Requests that arrive while
 others run join the batch at the next such boundary, so their work overlaps on

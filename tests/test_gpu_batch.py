@@ -1,5 +1,8 @@
 """Batched form: one set of launches for many scenes, checked against the same
-scenes alone (bit-exact, deterministic mode) and against the oracle."""
+scenes alone (bit-exact, deterministic mode) and against the oracle.  The
+launch knobs (graphs, grid size, XCD split, vertex workgroup) belong to the
+phased kernels, so those cases turn the persistent substep off; the default
+options take it for these small members."""
 import numpy as np
 import pytest
 
@@ -13,9 +16,13 @@ def _print_case(name, **kw):
     print(f"[case] {name} {kw}", flush=True)
 
 
-@pytest.mark.parametrize("opts", [dict(), dict(graph=False), dict(graph_substeps=5),
-                                  dict(xcd_split=False, vertex_block=128, grid_pct=50),
-                                  dict(grid_pct=1), dict(grid_pct=1, graph=False)])
+NEVER = dict(persistent="never")
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(persistent="always"), NEVER,
+                                  dict(graph=False, **NEVER), dict(graph_substeps=5, **NEVER),
+                                  dict(xcd_split=False, vertex_block=128, grid_pct=50, **NEVER),
+                                  dict(grid_pct=1, **NEVER), dict(grid_pct=1, graph=False, **NEVER)])
 def test_batch_members_match_alone_bitwise(opts):
     _print_case("batch_vs_alone", **opts)
     scenes = batched_scenes(n_scenes=6, seed=3, n_range=(3, 9), iters=20)
@@ -25,6 +32,8 @@ def test_batch_members_match_alone_bitwise(opts):
         s.solve()
         s.solve()
     with SceneBatch(mixed, **opts) as b:
+        assert b.path == ("graph" if opts.get("graph", True) else "kernels") \
+            if opts.get("persistent") == "never" else "persistent"
         if opts.get("grid_pct") == 1:  # workgroups walk patches of several scenes
             assert b.grid()["max_scenes_per_range"] >= 2, b.grid()
         b.frame(2)
@@ -36,11 +45,12 @@ def test_batch_members_match_alone_bitwise(opts):
         assert m.current_frame == 2
 
 
-def test_batch_member_matches_oracle():
-    _print_case("batch_vs_oracle")
+@pytest.mark.parametrize("persistent", ["auto", "never"])
+def test_batch_member_matches_oracle(persistent):
+    _print_case("batch_vs_oracle", persistent=persistent)
     scenes = batched_scenes(n_scenes=4, seed=5, n_range=(3, 8), iters=30)
     solvers = [make_solver(s) for s in scenes]
-    with SceneBatch(solvers) as b:
+    with SceneBatch(solvers, persistent=persistent) as b:
         b.frame(3)
         b.synchronize()
     for sc, s in zip(scenes, solvers):
@@ -50,13 +60,14 @@ def test_batch_member_matches_oracle():
         assert err <= 1e-4 * sc.diag, (sc.name, err)
 
 
-def test_batch_substeps_and_mixed_sizes():
-    _print_case("batch_substeps")
+@pytest.mark.parametrize("persistent", ["auto", "never"])
+def test_batch_substeps_and_mixed_sizes(persistent):
+    _print_case("batch_substeps", persistent=persistent)
     scenes = batched_scenes(n_scenes=3, seed=7, n_range=(2, 12), iters=10) + [resting_cube(n=4)]
     scenes[-1].params.update(substeps_num=5, constraints_iter=10)
     solvers = [make_solver(s) for s in scenes]
     ref = [make_solver(s) for s in scenes]
-    with SceneBatch(solvers) as b:
+    with SceneBatch(solvers, persistent=persistent) as b:
         b.substep(3)
         b.synchronize()
         assert all(s.current_substep == 2 for s in solvers)
@@ -70,17 +81,18 @@ def test_batch_substeps_and_mixed_sizes():
         assert np.array_equal(a.mesh.verts.x.to_numpy(), m.mesh.verts.x.to_numpy())
 
 
-@pytest.mark.parametrize("graph,grid_pct", [(False, 100), (True, 100), (True, 1)])
-def test_set_members_matches_fresh_batches(graph, grid_pct):
+@pytest.mark.parametrize("graph,grid_pct,persistent", [(False, 100, "never"), (True, 100, "never"),
+                                                       (True, 1, "never"), (True, 100, "auto")])
+def test_set_members_matches_fresh_batches(graph, grid_pct, persistent):
     """A batch re-targeted at other members (growing, shrinking and reordering
     its list) steps each scene exactly as plain solves do."""
-    _print_case("set_members", graph=graph, grid_pct=grid_pct)
+    _print_case("set_members", graph=graph, grid_pct=grid_pct, persistent=persistent)
     scenes = batched_scenes(n_scenes=5, seed=11, n_range=(3, 10), iters=15)
     alone = [make_solver(s) for s in scenes]
     mixed = [make_solver(s) for s in scenes]
     plan = [[0, 1], [1, 2, 3, 4], [4], [3, 0, 2]]  # members per substep round
     done = [0] * len(scenes)
-    with SceneBatch([mixed[0]], graph=graph, grid_pct=grid_pct) as b:
+    with SceneBatch([mixed[0]], graph=graph, grid_pct=grid_pct, persistent=persistent) as b:
         for members in plan:
             b.set_members([mixed[k] for k in members])
             assert b.n_scenes == len(members)

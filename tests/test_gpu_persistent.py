@@ -4,13 +4,16 @@ against the phased kernels replayed from a graph: bit-identical positions,
 velocities, multipliers and per-iteration logs in deterministic mode, over
 contact, split mode with the convergence monitor, the iteration counts at the
 edges of the sequence, and the reference's demo (cow) case; in atomic mode
-against the oracle."""
+against the oracle.  Its batched form (the members of a batch side by side in
+one launch, each meeting at its own barriers) against the phased batch and
+the scenes alone, and its path selection."""
 import numpy as np
 import pytest
 
 import golden_cases
-from scenes import dropped_cube, make_oracle, make_solver, resting_cube
+from scenes import batched_scenes, dropped_cube, make_oracle, make_solver, resting_cube
 from test_gpu_parity import check_positions, check_residual_curve, residual_rows, run_pair
+from xpbd_gpu.batch import SceneBatch
 from xpbd_gpu.mesh import box_grid
 
 pytestmark = pytest.mark.gpu
@@ -110,3 +113,82 @@ def test_large_scene_falls_back_to_graph():
     scene = jittered_box(n=24, frames=1, iters=10)
     a, b = _pair(scene, 1)
     _same(a, b)
+
+
+def _mixed_batch_scenes():
+    """Four batched-configuration boxes of 3-12 cells per axis and the 16^3
+    cube resting on the plane (48 patches), 5 substeps x 20 iterations."""
+    scenes = batched_scenes(n_scenes=4, seed=13, n_range=(3, 12), iters=20)
+    cube = resting_cube(frames=2)
+    cube.params.update(substeps_num=5, constraints_iter=20)
+    return scenes + [cube]
+
+
+@pytest.mark.parametrize("persistent", ["auto", "always"])
+def test_batch_persistent_matches_phased_and_alone(persistent):
+    print(f"[case] batch persistent {persistent}", flush=True)
+    scenes = _mixed_batch_scenes()
+    alone = [make_solver(s, exec_path="graph") for s in scenes]
+    per = [make_solver(s) for s in scenes]
+    phased = [make_solver(s) for s in scenes]
+    for s in alone:
+        s.solve()
+        s.solve()
+    with SceneBatch(per, persistent=persistent) as b:
+        assert b.path == "persistent"
+        b.frame(1)
+        b.substep(3)
+        b.synchronize()
+        b.substep(2)
+        b.synchronize()
+    with SceneBatch(phased, persistent="never") as b:
+        assert b.path == "graph"
+        b.frame(2)
+        b.synchronize()
+    for a, p, q in zip(alone, per, phased):
+        for s in (p, q):
+            assert np.array_equal(a.mesh.verts.x.to_numpy(), s.mesh.verts.x.to_numpy())
+            assert np.array_equal(a.mesh.verts.v.to_numpy(), s.mesh.verts.v.to_numpy())
+            assert np.array_equal(a.mesh.cells.lambda_D.to_numpy(),
+                                  s.mesh.cells.lambda_D.to_numpy())
+            assert a.rho_hat == s.rho_hat
+            assert s.current_frame == 2
+
+
+def test_batch_persistent_path_selection():
+    print("[case] batch persistent path selection", flush=True)
+    small = [make_solver(s) for s in batched_scenes(n_scenes=3, seed=5, n_range=(3, 6), iters=5)]
+    with SceneBatch(small) as b:
+        assert b.path == "persistent"
+    with SceneBatch(small, persistent="never") as b:
+        assert b.path == "graph"
+    with SceneBatch(small, persistent="never", graph=False) as b:
+        assert b.path == "kernels"
+    # a member of more than 96 patches: automatic selection keeps the phased
+    # kernels, "always" takes the persistent substep while all members fit
+    big_scene = batched_scenes(n_scenes=1, seed=5, n_range=(22, 22), iters=5)[0]
+    big = make_solver(big_scene)
+    with SceneBatch([big], persistent="never") as b:
+        groups = (b.n_patches + 1) // 2
+    assert 48 < groups <= 128, groups
+    with SceneBatch([big] + small) as b:
+        assert b.path == "graph"
+    ref = make_solver(big_scene, exec_path="graph")
+    with SceneBatch([big] + small, persistent="always") as b:
+        assert b.path == "persistent"
+        b.substep(2)
+        b.synchronize()
+    for _ in range(2):
+        sub_dt = ref.dt / ref.substeps_num
+        ref.apply_external_forces(sub_dt)
+        ref.solve_step(sub_dt)
+        ref.advance(sub_dt, ref.damping)
+    assert np.array_equal(ref.mesh.verts.x.to_numpy(), big.mesh.verts.x.to_numpy())
+    # more workgroups than the device holds at once: "always" refuses before
+    # launching anything, the automatic choice falls back
+    bigs = [make_solver(big_scene) for _ in range(6)]
+    with SceneBatch(bigs) as b:
+        assert b.path == "graph"
+    with SceneBatch(bigs, persistent="always") as b:
+        with pytest.raises(ValueError):
+            b.substep(1)

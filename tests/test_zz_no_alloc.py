@@ -24,16 +24,22 @@ def _noalloc(batch, solvers):
     return _abi.load().xpbd_batch_set_members_noalloc(batch._h, _handles(solvers), len(solvers))
 
 
-def test_noalloc_retarget_matches_allocating_retarget():
-    print("[case] noalloc_vs_set_members", flush=True)
+@pytest.mark.parametrize("persistent", ["auto", "never"])
+def test_noalloc_retarget_matches_allocating_retarget(persistent):
+    """With "auto" these small members take the persistent substep, whose
+    workgroup list then lives in the array too; with "never" the phased
+    kernels."""
+    print(f"[case] noalloc_vs_set_members {persistent}", flush=True)
     scenes = batched_scenes(n_scenes=6, seed=21, n_range=(3, 10), iters=15)
     a = [make_solver(s) for s in scenes]
     b = [make_solver(s) for s in scenes]
     rounds = [[0, 1, 2, 3, 4, 5], [4, 1]]  # the second call with a smaller membership
-    with SceneBatch([a[0]], graph=False) as ba, SceneBatch([b[0]], graph=False) as bb:
+    opts = dict(graph=False, persistent=persistent)
+    with SceneBatch([a[0]], **opts) as ba, SceneBatch([b[0]], **opts) as bb:
         for members in rounds:
             ba.set_members([a[k] for k in members])
             _abi.check(_noalloc(bb, [b[k] for k in members]))
+            assert bb.path == ("persistent" if persistent == "auto" else "kernels")
             ba.substep(3)
             _abi.check(_abi.load().xpbd_batch_substep(bb._h))
             _abi.check(_abi.load().xpbd_batch_substep(bb._h))
