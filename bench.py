@@ -104,6 +104,98 @@ def summarize_pmc(folder: str, copy_mib: int = 512) -> str:
     return "\n".join(lines) + "\n"
 
 
+COUNTER_SUMMARY = os.path.join(ROOT, "profiles", "iteration_kernels_jittered_box64_counters.txt")
+COUNTER_PASSES = {
+    "valu": "SQ_INSTS_VALU SQ_ACTIVE_INST_VALU SQ_INSTS_VALU_FMA_F32 SQ_INSTS_VALU_TRANS_F32 "
+            "SQ_INSTS_SALU SQ_WAVES SQ_BUSY_CYCLES SQ_WAVE_CYCLES GRBM_GUI_ACTIVE",
+    "lds": "SQ_INSTS_LDS SQ_ACTIVE_INST_LDS SQ_LDS_BANK_CONFLICT SQ_LDS_IDX_ACTIVE "
+           "SQ_WAIT_INST_LDS SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_INST_CYCLES_VMEM_RD GRBM_GUI_ACTIVE",
+}
+COUNTER_COMMANDS = "\n".join(
+    [f"rocprofv3 --pmc {c} --output-format csv -d $D/{k} -o run -- python3 bench.py --steps 1 "
+     "--warmup 0 --no-cpu-baseline --roofline-frames 0 --plain-only"
+     for k, c in COUNTER_PASSES.items()] +
+    [PMC_COMMANDS.splitlines()[i] for i in (1, 2, 3, 4)] +
+    ["rocprofv3 --kernel-trace --stats --output-format csv -d $D/stats -o run -- python3 bench.py "
+     "--steps 3 --warmup 1 --no-cpu-baseline --plain-only",
+     "python3 bench.py --summarize-counters $D"])
+
+
+def summarize_counters(folder: str, copy_mib: int = 512) -> str:
+    """The three kernels of a Jacobi iteration of the 64^3 jittered box, side
+    by side: average duration (kernel trace), algorithmic and measured HBM
+    bytes per launch with the fraction of 8 TB/s each gives, VALU issue
+    (wave instructions per launch against one per CU per cycle over the
+    launch's busy cycles), and LDS bank-conflict stalls."""
+    import csv
+
+    def per_kernel(path):
+        out = {}
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                name = r["Kernel_Name"].split("(")[0].replace("void ", "").strip()
+                out.setdefault(name, {}).setdefault(r["Counter_Name"], []).append(
+                    float(r["Counter_Value"]))
+        return {k: {c: float(np.mean(v)) for c, v in d.items()} for k, d in out.items()}
+
+    cnt = {}
+    for k in COUNTER_PASSES:
+        for name, d in per_kernel(os.path.join(folder, k, "run_counter_collection.csv")).items():
+            cnt.setdefault(name, {}).update(d)
+    with open(os.path.join(folder, "valu", "run_agent_info.csv")) as fh:
+        agent = next(r for r in csv.DictReader(fh) if r.get("Name", "").startswith("gfx"))
+    n_cu, n_se = int(agent["Cu_Count"]), int(agent["Array_Count"])
+    f = _pmc_per_kernel(os.path.join(folder, "fetch", "run_counter_collection.csv"))
+    w = _pmc_per_kernel(os.path.join(folder, "write", "run_counter_collection.csv"))
+    cf = _pmc_per_kernel(os.path.join(folder, "cfetch", "run_counter_collection.csv"))
+    cw = _pmc_per_kernel(os.path.join(folder, "cwrite", "run_counter_collection.csv"))
+    ck = next(k for k in cf if "copyBuffer" in k)
+    f_scale = copy_mib * 1024.0 / float(np.mean(cf[ck]))
+    w_scale = copy_mib * 1024.0 / float(np.mean(cw[ck]))
+    with open(os.path.join(folder, "stats", "run_kernel_stats.csv")) as fh:
+        dur = {r["Name"].split("(")[0].replace("void ", "").strip(): float(r["AverageNs"]) / 1e3
+               for r in csv.DictReader(fh)}
+    T, V, iters = 1572864, 274625, 100
+    alg = {"xpbd::k_patch_project<false>": b_alg_project(T, V, iters),
+           "xpbd::k_patch_average": 40 * V, "xpbd::k_patch_residual": 64 * T + 12 * V}
+    lines = ["# The kernels of one Jacobi iteration of the 64^3 jittered box (1 572 864 tets,",
+             "# 274 625 vertices), from rocprofv3 counter passes over one frame (1000 launches",
+             f"# each) on {agent['Name']} ({n_cu} CUs, {n_se} shader engines).  Made by:",
+             "#   cd <repo root>; export TMPDIR=/tmp; D=<output folder>"]
+    lines += ["#   " + c for c in COUNTER_COMMANDS.splitlines()]
+    lines += ["#",
+              "# alg MB: the algorithmic bytes per launch (136 T + 140 V per iteration, split",
+              "#   72 T + 28 V + 60 V (it-1)/it | 40 V | 64 T + 12 V); meas MB: FETCH_SIZE +",
+              "#   WRITE_SIZE scaled by a plain copy of known size; frac: MB / us against 8 TB/s.",
+              "# VALU/tet: SQ_INSTS_VALU x 64 / T (lane instructions per tet); VALU issue: the",
+              "#   wave instructions at one per CU per cycle over the launch's busy cycles",
+              "#   (SQ_BUSY_CYCLES per shader engine); clock: those cycles over the duration.",
+              "# LDS conflict: SQ_LDS_BANK_CONFLICT per CU over the busy cycles, and conflict",
+              "#   cycles per access cycle (SQ_LDS_BANK_CONFLICT / (SQ_LDS_IDX_ACTIVE - it)).",
+              "",
+              f"{'kernel':28s} {'us':>6s} {'alg MB':>7s} {'frac':>5s} {'meas MB':>7s} {'frac':>5s}"
+              f" {'VALU/tet':>8s} {'FMA%':>5s} {'VALU issue':>10s} {'clock':>6s}"
+              f" {'LDS conflict':>12s} {'per acc':>7s}"]
+    for k in ("xpbd::k_patch_project<false>", "xpbd::k_patch_residual", "xpbd::k_patch_average"):
+        c = cnt.get(k, {})
+        us = dur.get(k, float("nan"))
+        meas = (float(np.mean(f[k])) * f_scale + float(np.mean(w[k])) * w_scale) * 1024.0
+        busy = c.get("SQ_BUSY_CYCLES", 0.0) / n_se
+        valu = c.get("SQ_INSTS_VALU", 0.0)
+        conf = c.get("SQ_LDS_BANK_CONFLICT", 0.0)
+        idx = c.get("SQ_LDS_IDX_ACTIVE", 0.0)
+        lines.append(
+            f"{k:28s} {us:6.2f} {alg[k] / 1e6:7.1f} {alg[k] / us / 1e6 / 8:5.2f}"
+            f" {meas / 1e6:7.1f} {meas / us / 1e6 / 8:5.2f} {valu * 64 / T:8.0f}"
+            f" {100 * c.get('SQ_INSTS_VALU_FMA_F32', 0.0) / max(valu, 1):5.1f}"
+            f" {valu / n_cu / max(busy, 1):10.2f} {busy / us / 1e3:5.2f}G"
+            f" {conf / n_cu / max(busy, 1):12.2f} {conf / max(idx - conf, 1):7.2f}")
+    lines += ["", "raw counters per launch:"]
+    for k in ("xpbd::k_patch_project<false>", "xpbd::k_patch_residual", "xpbd::k_patch_average"):
+        lines.append(f"  {k}: " + ", ".join(f"{n} {v:.0f}" for n, v in sorted(cnt.get(k, {}).items())))
+    return "\n".join(lines) + "\n"
+
+
 STATS_COMMAND = ("cd <repo root>; export TMPDIR=/tmp; rocprofv3 --kernel-trace --stats "
                  "--output-format csv -d <D> -o run -- python3 bench.py --steps 3 --warmup 1 "
                  "--no-cpu-baseline --plain-only; python3 bench.py --summarize-stats <D>")
@@ -502,6 +594,8 @@ def main():
                     help="only run a plain device copy of this size (counter calibration)")
     ap.add_argument("--summarize-pmc", default="",
                     help="only summarize the FETCH_SIZE/WRITE_SIZE passes in this folder")
+    ap.add_argument("--summarize-counters", default="",
+                    help="only summarize the counter passes of the iteration kernels in this folder")
     ap.add_argument("--summarize-stats", default="",
                     help="only summarize a rocprofv3 --kernel-trace --stats folder")
     ap.add_argument("--timing-replays", type=int, default=8,
@@ -521,6 +615,9 @@ def main():
         return
     if args.summarize_small_stats:
         sys.stdout.write(summarize_small_stats(args.summarize_small_stats))
+        return
+    if args.summarize_counters:
+        sys.stdout.write(summarize_counters(args.summarize_counters))
         return
     if args.summarize_pmc:
         sys.stdout.write(summarize_pmc(args.summarize_pmc))
