@@ -367,7 +367,7 @@ def batched_measurement(n_scenes: int, frames: int, tune: bool, sequential: bool
           file=sys.stderr, flush=True)
     opts, trials = {}, []
     if tune:
-        cands = [c for c in default_candidates(subs) if c["grid_pct"] == 100]
+        cands = [c for c in default_candidates(subs) if c.get("grid_pct", 100) == 100]
         opts, trials = autotune(solvers, candidates=cands)
     with SceneBatch(solvers, **opts) as b:
         b.frame(1)  # warm-up (graph capture)

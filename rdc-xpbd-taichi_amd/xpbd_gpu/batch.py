@@ -10,8 +10,10 @@ Scenes keep their own buffers, materials, SDFs and rho/omega state; the
 results of a scene are bit-identical alone and in any batch.
 
 `autotune` picks the batch's launch knobs (vertex workgroup size, persistent
-grid size, XCD-contiguous work split, substeps per graph) by timing them on the
-actual scenes, restoring the scenes' state after each trial.
+grid size, XCD-contiguous work split, substeps per graph, and whether a
+substep's iterations run in one persistent launch) by timing them on the
+actual scenes, restoring the scenes' state after each trial;
+`autotune_exec_path` does the same for one scene's execution path.
 """
 from __future__ import annotations
 
@@ -160,17 +162,23 @@ def _restore(solvers, snap):
 
 
 def default_candidates(substeps_num: int):
+    """The phased kernels' launch knobs, and the persistent substep (all of a
+    substep's iterations in one launch; skipped by `autotune` where it does
+    not apply)."""
     grid = itertools.product([128, 256, 512], [50, 100], [True, False],
                              sorted({1, int(substeps_num)}))
     # graph_substeps > 1 holds whole frames (dt == frame_dt)
-    return [dict(graph=True, vertex_block=vb, grid_pct=gp, xcd_split=xs, graph_substeps=gs)
-            for vb, gp, xs, gs in grid]
+    return [dict(graph=True, vertex_block=vb, grid_pct=gp, xcd_split=xs, graph_substeps=gs,
+                 persistent="never")
+            for vb, gp, xs, gs in grid] + [dict(persistent="always")]
 
 
 def autotune(solvers, candidates=None, substeps: int | None = None, warmup: int = 1):
     """Times each candidate option set on the scenes and returns
     (best_opts, [(opts, seconds_per_substep), ...]).  Scene state (positions,
-    velocities, rho_hat, counters) is restored after every trial."""
+    velocities, rho_hat, counters) is restored after every trial.  A candidate
+    that does not apply to these scenes (persistent="always" for scenes too
+    large to be resident together) is listed with an infinite time."""
     solvers = list(solvers)
     s0 = solvers[0]
     if candidates is None:
@@ -183,6 +191,9 @@ def autotune(solvers, candidates=None, substeps: int | None = None, warmup: int 
             n = substeps if substeps is not None else max(2, per)
             n = max(per, (n // per) * per)
             with SceneBatch(solvers, **opts) as b:
+                if opts.get("persistent") == "always" and b.path != "persistent":
+                    results.append((dict(opts), float("inf")))
+                    continue
                 # whole-frame graphs are replayed by frame(); single substeps by substep()
                 step = (lambda k: b.frame(k // per)) if per > 1 else b.substep
                 step(max(warmup, 0) * per)
@@ -196,6 +207,47 @@ def autotune(solvers, candidates=None, substeps: int | None = None, warmup: int 
     finally:
         _restore(solvers, snap)
     best = min(results, key=lambda r: r[1])[0]
+    return best, results
+
+
+def autotune_exec_path(solver, paths=("persistent", "graph", "kernels"), frames: int = 1,
+                       warmup: int = 1):
+    """Times one scene's execution paths, `frames` frames of solve() each after
+    `warmup`, and leaves the solver on the fastest.  The paths differ in
+    iterations per launch: "persistent" runs all of a substep's iterations in
+    one launch (where the scene is small enough, else it takes the graph),
+    "graph" and "kernels" one launch per kernel, replayed from a graph or not;
+    every path gives the same bits.  The scene's state, frame counters,
+    convergence history and solve timers are restored after each trial.
+    Returns (best_path, [(path, seconds_per_frame), ...])."""
+    snap = _snapshot([solver])
+    hist = len(solver.convergence_history)
+    timers = (solver.solve_compute_time_total, solver.solve_compute_calls,
+              solver.solve_compute_iterations)
+
+    def restore():
+        _restore([solver], snap)
+        del solver.convergence_history[hist:]
+        (solver.solve_compute_time_total, solver.solve_compute_calls,
+         solver.solve_compute_iterations) = timers
+
+    results = []
+    try:
+        for path in paths:
+            solver.exec_path = path
+            for _ in range(max(warmup, 0)):
+                solver.solve()
+            solver.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(frames):
+                solver.solve()
+            solver.synchronize()
+            results.append((path, (time.perf_counter() - t0) / max(1, frames)))
+            restore()
+    finally:
+        restore()
+    best = min(results, key=lambda r: r[1])[0]
+    solver.exec_path = best
     return best, results
 
 

@@ -206,6 +206,19 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         self._pushed = self._param_key()
         self._logging = None
 
+    @property
+    def exec_path(self) -> str:
+        """How a substep reaches the GPU: "auto", "persistent", "graph" or
+        "kernels" (xpbd_set_exec_path); every path gives the same bits."""
+        return self._exec_path
+
+    @exec_path.setter
+    def exec_path(self, path: str):
+        if path not in _PATHS:
+            raise ValueError(f"exec_path must be one of {sorted(_PATHS)}")
+        _abi.check(self._lib.xpbd_set_exec_path(self._h, _PATHS[path]))
+        self._exec_path = path
+
     def _param_key(self):
         g = tuple(float(x) for x in np.asarray(self.gravity[None], dtype=np.float64))
         sd = tuple(s.key() for s in self.sdfs)

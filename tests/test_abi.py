@@ -104,3 +104,21 @@ def test_out_of_range_vertex_index_is_rejected_before_gpu_work(bad):
     assert lib.xpbd_create(ctypes.byref(d), ctypes.byref(h)) == _abi.XPBD_EINVAL
     assert "tet 1 references vertex" in _abi.last_error()
     assert not h.value
+
+
+def test_batch_opts_defaults_and_tuning_candidates():
+    from xpbd_gpu.batch import _opts_struct, default_candidates
+    o = _abi.XpbdBatchOpts()
+    assert _abi.load().xpbd_batch_opts_init(ctypes.byref(o)) == 0
+    assert (o.graph, o.graph_substeps, o.vertex_block, o.grid_pct, o.xcd_split,
+            o.persistent) == (1, 1, 256, 100, 1, 1)
+    assert _opts_struct(dict(persistent="never")).persistent == 0
+    assert _opts_struct(dict(persistent="always")).persistent == 2
+    with pytest.raises(ValueError):
+        _opts_struct(dict(persistent="sometimes"))
+    cands = default_candidates(5)
+    # the phased knobs are timed on the phased kernels; one candidate runs every
+    # iteration of a substep in one persistent launch
+    assert [c for c in cands if c.get("persistent") == "always"] == [dict(persistent="always")]
+    assert all(c["persistent"] == "never" for c in cands if "grid_pct" in c)
+    assert {c["graph_substeps"] for c in cands if "grid_pct" in c} == {1, 5}

@@ -151,3 +151,34 @@ def test_autotune_restores_state():
     for s, x in zip(solvers, x0):
         assert np.array_equal(s.mesh.verts.x.to_numpy(), x)
         assert s.current_frame == 0
+
+
+def test_autotune_skips_persistent_where_it_does_not_apply():
+    """The persistent candidate is timed on small members and listed with an
+    infinite time where the members cannot all be resident at once."""
+    _print_case("autotune_persistent")
+    small = [make_solver(s) for s in batched_scenes(n_scenes=3, seed=9, n_range=(3, 8), iters=5)]
+    best, res = autotune(small, candidates=[dict(persistent="never"), dict(persistent="always")])
+    assert all(np.isfinite(t) for _, t in res)
+    big = [make_solver(s) for s in batched_scenes(n_scenes=6, seed=5, n_range=(22, 22), iters=5)]
+    best, res = autotune(big, candidates=[dict(persistent="never"), dict(persistent="always")])
+    assert best == dict(persistent="never")
+    assert np.isfinite(res[0][1]) and res[1][1] == float("inf")
+
+
+def test_autotune_exec_path_restores_state_and_keeps_the_bits():
+    _print_case("autotune_exec_path")
+    from xpbd_gpu.batch import autotune_exec_path
+    scene = resting_cube(n=6, frames=2)
+    s = make_solver(scene, monitor_convergence=True)
+    ref = make_solver(scene, monitor_convergence=True, exec_path="graph")
+    x0 = s.mesh.verts.x.to_numpy()
+    best, res = autotune_exec_path(s, frames=1)
+    assert best in ("persistent", "graph", "kernels") and s.exec_path == best
+    assert [p for p, _ in res] == ["persistent", "graph", "kernels"]
+    assert np.array_equal(s.mesh.verts.x.to_numpy(), x0)
+    assert s.current_frame == 0 and s.convergence_history == [] and s.solve_compute_calls == 0
+    for _ in range(2):
+        s.solve()
+        ref.solve()
+    assert np.array_equal(s.mesh.verts.x.to_numpy(), ref.mesh.verts.x.to_numpy())
