@@ -224,6 +224,12 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets);
  *                        otherwise. */
 #define XPBD_PATH_AUTO 3
 int xpbd_set_exec_path(xpbd_solver* s, int32_t path);
+/* No counterpart in the reference: how long one scene barrier of the
+ * persistent substep waits before it gives up (default 2 000 000 us), for
+ * callers that share the GPU with other work and for tests of the XPBD_EHIP
+ * path (0: give up at once unless every workgroup has already arrived).
+ * Microseconds in [0, 1e9]. */
+int xpbd_set_barrier_timeout(xpbd_solver* s, int64_t microseconds);
 
 /* Kernel timing (no reference counterpart).  mode 0: off.  mode 1: every
  * launch of the tet projection kernel is timed by HIP events on the solver's
@@ -309,6 +315,8 @@ int xpbd_batch_synchronize(xpbd_batch* b);
 /* The path the batch's next substeps take with its current members:
  * XPBD_PATH_PERSISTENT, XPBD_PATH_GRAPH or XPBD_PATH_KERNELS. */
 int xpbd_batch_path(xpbd_batch* b, int32_t* path);
+/* The batch's xpbd_set_barrier_timeout (no counterpart in the reference). */
+int xpbd_batch_set_barrier_timeout(xpbd_batch* b, int64_t microseconds);
 int xpbd_batch_get_stream(xpbd_batch* b, void** stream);
 int xpbd_batch_info(xpbd_batch* b, int32_t* n_scenes, int64_t* n_verts, int64_t* n_tets,
                     int32_t* n_patches);

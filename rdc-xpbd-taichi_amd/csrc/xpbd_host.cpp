@@ -198,6 +198,11 @@ int validate_sdfs(const xpbd_sdf* sdfs, int n) {
 // any sum, so results are bit-identical.  Returns the vertices in slot order.
 constexpr int kLdsPassLanes = 8;
 
+// The default wait of one scene barrier of the persistent substep before it
+// gives up: 2 s of the 100 MHz wall clock.
+constexpr unsigned long long kPersistTimeoutTicks = 200000000ull;
+constexpr unsigned long long kWallClockTicksPerUs = 100ull;
+
 std::vector<int> lds_slot_order(const std::vector<int>& verts, const std::vector<int>& tets,
                                 int t0, int t1, std::vector<int>& local) {
   const int n = int(verts.size());
@@ -438,6 +443,7 @@ struct xpbd_solver {
   DevBuf<unsigned> sync;
   int persist_capacity = 0;
   bool persist_pending = false;
+  unsigned long long persist_timeout = kPersistTimeoutTicks;
   int n_patches = 0;
   int layout = XPBD_LAYOUT_PATCH;
   DevBuf<float> valence;
@@ -962,10 +968,8 @@ int fetch_log(xpbd_solver* s) {
   return XPBD_OK;
 }
 
-// Persistent substep limits: the wait of one scene barrier (ticks of the
-// 100 MHz wall clock) and the largest scene it is used for.
-constexpr unsigned long long kPersistTimeoutTicks = 200000000ull;  // 2 s
-constexpr int kPersistMaxGroups = 128;                             // 256 patches
+// The largest scene the persistent substep is used for.
+constexpr int kPersistMaxGroups = 128;  // 256 patches
 // XPBD_PATH_AUTO takes the persistent substep up to this many groups: a scene
 // barrier costs 1.6 us at 8 workgroups, 2.6 us at 24, 3.8 us at 48 (agent-scope
 // release and acquire per workgroup), and two per iteration beat the phased
@@ -997,7 +1001,7 @@ int run_substep(xpbd_solver* s, double sub_dt) {
                        !s->timing && persistent_ok(s);
   if (persist) {
     HIP_TRY(xpbd::launch_substep_persistent(s->table.p, s->pitems.p, s->host_scene.n_partials,
-                                            kPersistTimeoutTicks, s->stream));
+                                            s->persist_timeout, s->stream));
     s->persist_pending = true;
     return fetch_log(s);
   }
@@ -1623,6 +1627,14 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets) {
   return XPBD_OK;
 }
 
+int xpbd_set_barrier_timeout(xpbd_solver* s, int64_t microseconds) {
+  if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
+  if (microseconds < 0 || microseconds > int64_t(1e9))
+    return set_error(XPBD_EINVAL, "barrier timeout must be in [0, 1e9] microseconds");
+  s->persist_timeout = (unsigned long long)microseconds * kWallClockTicksPerUs;
+  return XPBD_OK;
+}
+
 int xpbd_set_exec_path(xpbd_solver* s, int32_t path) {
   if (!s) return set_error(XPBD_EINVAL, "solver handle is NULL");
   if (path != XPBD_PATH_KERNELS && path != XPBD_PATH_GRAPH && path != XPBD_PATH_PERSISTENT &&
@@ -1779,6 +1791,7 @@ struct xpbd_batch {
   DevBuf<unsigned> perr;
   int persist_capacity = 0;
   bool persist_pending = false;
+  unsigned long long persist_timeout = kPersistTimeoutTicks;
 
   ~xpbd_batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -2049,7 +2062,7 @@ int batch_run(xpbd_batch* b, int n_substeps) {
   if (batch_persistent(b)) {
     for (int k = 0; k < n_substeps; ++k) {
       HIP_TRY(xpbd::launch_batch_substep_persistent(b->table_d, b->items_d, b->pgroups_d,
-                                                    b->n_pgroups, b->perr.p, kPersistTimeoutTicks,
+                                                    b->n_pgroups, b->perr.p, b->persist_timeout,
                                                     b->stream));
       b->persist_pending = true;
       batch_count_substep(b);
@@ -2217,6 +2230,14 @@ int xpbd_batch_synchronize(xpbd_batch* b) {
     return set_error(XPBD_EHIP, "persistent batch substep: a scene barrier timed out (its "
                                 "workgroups were not all resident); the results are invalid");
   }
+  return XPBD_OK;
+}
+
+int xpbd_batch_set_barrier_timeout(xpbd_batch* b, int64_t microseconds) {
+  if (!b) return set_error(XPBD_EINVAL, "batch is NULL");
+  if (microseconds < 0 || microseconds > int64_t(1e9))
+    return set_error(XPBD_EINVAL, "barrier timeout must be in [0, 1e9] microseconds");
+  b->persist_timeout = (unsigned long long)microseconds * kWallClockTicksPerUs;
   return XPBD_OK;
 }
 

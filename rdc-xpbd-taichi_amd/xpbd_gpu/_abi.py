@@ -59,7 +59,7 @@ EXPORTED = (
     "xpbd_batch_destroy", "xpbd_batch_set_members", "xpbd_batch_set_members_noalloc",
     "xpbd_batch_substep", "xpbd_batch_frame",
     "xpbd_batch_synchronize", "xpbd_batch_get_stream", "xpbd_batch_info", "xpbd_batch_grid",
-    "xpbd_batch_path",
+    "xpbd_batch_path", "xpbd_set_barrier_timeout", "xpbd_batch_set_barrier_timeout",
 )
 
 
@@ -217,6 +217,8 @@ def load() -> ctypes.CDLL:
         "xpbd_batch_grid": [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                             ctypes.POINTER(ctypes.c_int32)],
         "xpbd_batch_path": [vp, ctypes.POINTER(ctypes.c_int32)],
+        "xpbd_set_barrier_timeout": [vp, ctypes.c_int64],
+        "xpbd_batch_set_barrier_timeout": [vp, ctypes.c_int64],
         "xpbd_patch_stats": [fp, ctypes.POINTER(ctypes.c_int32), i32, i32,
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64),
                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)],

@@ -97,6 +97,11 @@ class SceneBatch:
         return {_abi.PATH_PERSISTENT: "persistent", _abi.PATH_GRAPH: "graph",
                 _abi.PATH_KERNELS: "kernels"}[v.value]
 
+    def set_barrier_timeout(self, microseconds: int):
+        """xpbd_batch_set_barrier_timeout: the persistent substep's barrier wait
+        (default 2 s)."""
+        _abi.check(self._lib.xpbd_batch_set_barrier_timeout(self._h, int(microseconds)))
+
     def set_members(self, solvers):
         """Re-targets the batch at another list of scenes (xpbd_batch_set_members):
         the stream and large-enough device buffers are kept; queued work is

@@ -219,6 +219,12 @@ class XPBDNeoHookeanSolver(XPBDSolverBase):
         _abi.check(self._lib.xpbd_set_exec_path(self._h, _PATHS[path]))
         self._exec_path = path
 
+    def set_barrier_timeout(self, microseconds: int):
+        """How long a scene barrier of the persistent substep waits before it
+        gives up and the next synchronizing call raises (xpbd_set_barrier_timeout;
+        default 2 s)."""
+        _abi.check(self._lib.xpbd_set_barrier_timeout(self._h, int(microseconds)))
+
     def _param_key(self):
         g = tuple(float(x) for x in np.asarray(self.gravity[None], dtype=np.float64))
         sd = tuple(s.key() for s in self.sdfs)

@@ -192,3 +192,50 @@ def test_batch_persistent_path_selection():
     with SceneBatch(bigs, persistent="always") as b:
         with pytest.raises(ValueError):
             b.substep(1)
+
+
+def test_barrier_timeout_is_reported_and_the_solver_recovers():
+    """A scene barrier that gives up (forced here by a zero wait, so every
+    workgroup still leaves its loops) is reported by the next synchronizing
+    call; the error word and the barrier counters are reset, so the solver
+    continues normally from a saved state."""
+    from xpbd_gpu._abi import XpbdError
+    print("[case] persistent barrier timeout", flush=True)
+    scene = resting_cube(frames=2)  # 48 patches: 24 workgroups
+    s = make_solver(scene, exec_path="persistent")
+    ref = make_solver(scene, exec_path="persistent")
+    st = s.save_state()
+    s.set_barrier_timeout(0)
+    with pytest.raises(XpbdError, match="timed out"):
+        s.solve()
+    s.set_barrier_timeout(2_000_000)
+    s.load_state(st)
+    for _ in range(2):
+        s.solve()
+        ref.solve()
+    assert np.array_equal(s.mesh.verts.x.to_numpy(), ref.mesh.verts.x.to_numpy())
+
+
+def test_batch_barrier_timeout_is_reported_and_the_batch_recovers():
+    from xpbd_gpu._abi import XpbdError
+    print("[case] batch persistent barrier timeout", flush=True)
+    scenes = _mixed_batch_scenes()
+    per = [make_solver(s) for s in scenes]
+    ref = [make_solver(s) for s in scenes]
+    states = [s.save_state() for s in per]
+    with SceneBatch(per) as b:
+        assert b.path == "persistent"
+        b.set_barrier_timeout(0)
+        b.substep(1)
+        with pytest.raises(XpbdError, match="timed out"):
+            b.synchronize()
+        b.set_barrier_timeout(2_000_000)
+        for s, st in zip(per, states):
+            s.load_state(st)
+        b.frame(1)
+        b.synchronize()
+    with SceneBatch(ref, persistent="never") as b:
+        b.frame(1)
+        b.synchronize()
+    for a, r in zip(per, ref):
+        assert np.array_equal(a.mesh.verts.x.to_numpy(), r.mesh.verts.x.to_numpy())
