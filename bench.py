@@ -190,6 +190,12 @@ def summarize_counters(folder: str, copy_mib: int = 512) -> str:
             f" {100 * c.get('SQ_INSTS_VALU_FMA_F32', 0.0) / max(valu, 1):5.1f}"
             f" {valu / n_cu / max(busy, 1):10.2f} {busy / us / 1e3:5.2f}G"
             f" {conf / n_cu / max(busy, 1):12.2f} {conf / max(idx - conf, 1):7.2f}")
+    c = cnt.get("xpbd::k_patch_project<false>", {})
+    busy = c.get("SQ_BUSY_CYCLES", 0.0) / n_se
+    lines += ["", "valu_issue_frac xpbd::k_patch_project<false>: "
+                  f"{c.get('SQ_INSTS_VALU', 0.0) / n_cu / max(busy, 1):.3f}",
+              "lds_conflict_stall_frac xpbd::k_patch_project<false>: "
+              f"{c.get('SQ_LDS_BANK_CONFLICT', 0.0) / n_cu / max(busy, 1):.3f}"]
     lines += ["", "raw counters per launch:"]
     for k in ("xpbd::k_patch_project<false>", "xpbd::k_patch_residual", "xpbd::k_patch_average"):
         lines.append(f"  {k}: " + ", ".join(f"{n} {v:.0f}" for n, v in sorted(cnt.get(k, {}).items())))
@@ -290,6 +296,25 @@ def committed_traffic(path: str = PMC_SUMMARY):
     except OSError:
         pass
     return None
+
+
+def committed_counters(path: str = COUNTER_SUMMARY):
+    """The projection kernel's VALU issue and LDS bank-conflict stall, as
+    fractions of its busy cycles, from the committed counter summary (None
+    when it is absent)."""
+    out = {}
+    try:
+        with open(path) as fh:
+            for line in fh:
+                for key in ("valu_issue_frac", "lds_conflict_stall_frac"):
+                    if line.startswith(key + " xpbd::k_patch_project"):
+                        out[key] = float(line.rsplit(":", 1)[1])
+    except OSError:
+        return None
+    if not out:
+        return None
+    out["source"] = os.path.relpath(path, ROOT)
+    return out
 
 
 def _cpu_model() -> str:
@@ -697,13 +722,17 @@ def main():
     b_proj = b_alg_project(T, V, iters)
     ach = b_proj / avg_s / 1e9 if avg_s > 0 else 0.0
     traffic = committed_traffic() if (args.n == 64 and args.layout == "patch") else None
+    counters = committed_counters() if (args.n == 64 and args.layout == "patch") else None
     roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBPS, 4), "traffic": traffic,
                 "kernel": "xpbd::k_patch_project<false>" if args.layout == "patch"
                           else "xpbd::k_project",
                 "avg_us": round(avg_s * 1e6, 2), "launches": launches,
                 "avg_us_one_launch_per_event_pair": round(avg_single_s * 1e6, 2),
-                "bytes_per_launch": b_proj}
+                "bytes_per_launch": b_proj,
+                # the same kernel by rocprofv3 counters: the fraction of its busy
+                # cycles that issue a VALU instruction, and LDS conflict stalls
+                "counters": counters}
 
     out = {
         "metric": "tet-iterations/s & HBM GB/s at 1.57M tets; p99 step latency batched vs plain",
