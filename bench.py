@@ -348,7 +348,7 @@ def _cpu_leg(scene, substeps: int, threads: int) -> dict:
 
 def cpu_baseline(scene, substeps: int, threads: int) -> dict:
     """The CPU restatement of the reference kernel sequence (the f32 oracle,
-    OpenMP over tets, `threads` threads) on the GPU box's host: a bounded
+    OpenMP over tets, `threads` threads) on the host of the benchmarked GPU: a bounded
     sample of the headline mesh (`substeps` substeps of the 64^3 box) and, as
     a second leg, the 16^3 cube's 10 frames in full."""
     from xpbd_gpu.scenes import resting_cube
@@ -665,7 +665,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
