@@ -84,3 +84,69 @@ def test_admission_that_cannot_fit_changes_nothing(monkeypatch):
     # once scene 0 is idle, admission evicts 0 (least recently used) and builds it
     assert srv.make_resident(2)
     assert list(srv.resident) == [1, 2] and srv.evictions == 1
+
+
+class _Spec:
+    def __init__(self, substeps):
+        self.params = {"substeps_num": substeps}
+
+
+class _StepSolver(_FakeSolver):
+    n_tets, constraints_iter = 100, 2
+
+    def __init__(self, spec):
+        super().__init__(spec)
+        self.substeps_num = spec.params["substeps_num"]
+
+
+class _FakeBatch:
+    """Records what the serving loop enqueues and when it waits."""
+    log = []
+
+    def __init__(self, members, **opts):
+        _FakeBatch.log.append(("create", len(members)))
+
+    def set_members(self, members):
+        _FakeBatch.log.append(("members", len(members)))
+
+    def substep(self, n):
+        _FakeBatch.log.append(("substep", n))
+
+    def synchronize(self):
+        _FakeBatch.log.append(("wait",))
+
+    def close(self):
+        pass
+
+
+def _stepping_server(monkeypatch, max_run=None):
+    import xpbd_gpu.serve as serve
+    monkeypatch.setattr(serve, "make_solver", lambda spec, **kw: _StepSolver(spec))
+    monkeypatch.setattr(serve, "scene_bytes", lambda s: 1)
+    monkeypatch.setattr(serve, "SceneBatch", _FakeBatch)
+    _FakeBatch.log = []
+    return serve.Server([_Spec(3), _Spec(3)], max_run=max_run)
+
+
+def test_loop_enqueues_up_to_the_next_completion_then_waits_once(monkeypatch):
+    from xpbd_gpu.serve import Request
+    srv = _stepping_server(monkeypatch)
+    reqs = [Request(0, 0, 2, 0.0), Request(1, 1, 1, 0.0), Request(2, 0, 1, 0.0)]
+    done = {d.rid: d for d in srv.run(reqs)}
+    # request 1 (one frame of 3 substeps) ends the first run; request 0 runs its
+    # second frame alone; request 2 follows on the same scene (same membership)
+    assert _FakeBatch.log == [("create", 2), ("substep", 3), ("wait",), ("members", 1),
+                              ("substep", 3), ("wait",), ("substep", 3), ("wait",)]
+    assert srv.substeps_run == 9 and srv.syncs == 3 and srv.rebuilds == 2
+    assert done[1].finish <= done[0].finish <= done[2].start <= done[2].finish
+    assert all(d.latency >= d.service >= 0.0 for d in done.values())
+    assert done[0].tet_iterations == 100 * 2 * 2 * 3
+    assert srv.step_s == srv.launch_s + srv.wait_s
+
+
+def test_loop_run_length_is_capped(monkeypatch):
+    from xpbd_gpu.serve import Request
+    srv = _stepping_server(monkeypatch, max_run=1)
+    srv.run([Request(0, 0, 2, 0.0)])
+    assert [e for e in _FakeBatch.log if e[0] == "substep"] == [("substep", 1)] * 6
+    assert srv.syncs == 6
