@@ -235,11 +235,14 @@ __device__ __forceinline__ float4 jacobi_from(const SceneDev& S, float4 acc, flo
   return make_float4(x.x, x.y, x.z, p.w);
 }
 
-// x_J of vertex v with delta_x summed from its patch slots.
+// x_J of vertex v with delta_x summed from its patch slots.  In block mode
+// with deterministic accumulation every corner adds a count of exactly 1, so
+// the slots' summed count is the valence (the same float) and the valence is
+// not read; LDS atomics in block mode leave the count at 0.
 __device__ __forceinline__ float4 jacobi_average_slots(const SceneDev& S, int v, int it) {
   const float4 acc = slot_sum(S, S.vslot_off[v], S.vslot_off[v + 1]);
   const float4 p = S.fused ? S.ybuf[it % 3][v] : S.pos[v];
-  return jacobi_from(S, acc, S.valence[v], p);
+  return jacobi_from(S, acc, S.block_mode && !S.csr ? S.valence[v] : acc.w, p);
 }
 
 // apply_external_forces (xpbd_base.py:139-146) and the collider projections
