@@ -13,6 +13,7 @@ oracle timed on the host cores over a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -409,6 +410,8 @@ def batched_measurement(n_scenes: int, frames: int, tune: bool, sequential: bool
     workload = (f"{len(solvers)} box scenes, per-axis resolution uniform in 6..40 (seed 0), "
                 f"{T} tets in total, {subs} substeps x {iters} iterations per frame")
     if not sequential:
+        for s in solvers:
+            s.close()
         return {"workload": workload, "tet_iterations_per_s": batched,
                 "frame_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                              "p99": float(np.percentile(lat, 99) * 1e3)},
@@ -424,6 +427,8 @@ def batched_measurement(n_scenes: int, frames: int, tune: bool, sequential: bool
         seq_lat.append(time.perf_counter() - s0)
     seq_el = time.perf_counter() - t0
     sequential = T * iters * subs / seq_el
+    for s in solvers:
+        s.close()
     return {
         "workload": workload,
         "tet_iterations_per_s": batched,
@@ -445,6 +450,9 @@ def _served_run(srv, reqs) -> dict:
     and where the host's time went (re-targeting, enqueueing, waiting)."""
     from xpbd_gpu.serve import summarize
 
+    # scenes of earlier measurements that are only reachable through reference
+    # cycles are freed (device frees synchronize) before the clock starts
+    gc.collect()
     srv.reset_counters()
     t0 = time.perf_counter()
     done = srv.run(reqs)
@@ -471,6 +479,7 @@ def _served_run(srv, reqs) -> dict:
 def _sequential_run(scenes, reqs, solvers) -> dict:
     from xpbd_gpu.serve import run_sequential, summarize
 
+    gc.collect()
     t0 = time.perf_counter()
     done = run_sequential(scenes, reqs, solvers=solvers)
     st = summarize(done, time.perf_counter() - t0)
@@ -761,6 +770,7 @@ def main():
                             "p99": float(np.percentile(lat, 99) * 1e3)},
         "roofline": roofline,
     }
+    solver.close()
     del solver
     if not args.plain_only:
         out["small_configs"] = small_configs_measurement()

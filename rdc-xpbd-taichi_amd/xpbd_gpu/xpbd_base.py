@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes
 import warnings
+import weakref
 
 import numpy as np
 
@@ -25,11 +26,40 @@ def calc_tet_volume(p0, p1, p2, p3):
     return np.einsum("...i,...i->...", np.cross(p1 - p0, p2 - p0), p3 - p0) / 6.0
 
 
-class _VertexAttr:
-    """A per-vertex attribute of the device mesh, read and written as numpy."""
+class _SolverRef:
+    """A weak reference to the owning solver: the mesh views must not keep
+    the solver alive, so that a dropped solver frees its device state at once
+    rather than whenever the cycle collector runs (a device free waits for the
+    device)."""
+
+    def __init__(self, solver):
+        self._ref = weakref.ref(solver)
+
+    @property
+    def _solver(self):
+        s = self._ref()
+        if s is None:
+            raise ReferenceError("the solver of this field has been freed")
+        return s
+
+
+def _weak_method(method):
+    """Calls `method` while its object is alive (for change callbacks held by
+    the object's own fields)."""
+    ref = weakref.WeakMethod(method)
+
+    def call(*args):
+        m = ref()
+        return None if m is None else m(*args)
+    return call
+
+
+class _VertexAttr(_SolverRef):
+    """A per-vertex attribute of the device mesh, read and written as numpy;
+    `getter(solver)` and `setter(solver, arr)` do the transfers."""
 
     def __init__(self, solver, getter, setter=None, width=3):
-        self._solver = solver
+        super().__init__(solver)
         self._getter = getter
         self._setter = setter
         self._width = width
@@ -39,17 +69,17 @@ class _VertexAttr:
         return (self._solver.n_verts,)
 
     def to_numpy(self) -> np.ndarray:
-        return self._getter()
+        return self._getter(self._solver)
 
     def from_numpy(self, arr):
         if self._setter is None:
             raise AttributeError("this attribute is read-only")
-        self._setter(arr)
+        self._setter(self._solver, arr)
 
 
-class _CellAttr:
+class _CellAttr(_SolverRef):
     def __init__(self, solver, getter):
-        self._solver = solver
+        super().__init__(solver)
         self._getter = getter
 
     @property
@@ -57,7 +87,7 @@ class _CellAttr:
         return (self._solver.n_tets,)
 
     def to_numpy(self) -> np.ndarray:
-        return self._getter()
+        return self._getter(self._solver)
 
 
 class _Elements:
@@ -80,11 +110,11 @@ class _Mesh:
         return self._rest.verts.copy()
 
 
-class _IndexField:
+class _IndexField(_SolverRef):
     """The `indices` field (xpbd_base.py:57-58): 4 faces x 3 ids per tet, uint32."""
 
     def __init__(self, solver):
-        self._solver = solver
+        super().__init__(solver)
         self._cache = None
 
     @property
@@ -113,7 +143,7 @@ class XPBDSolverBase:
         self.dt = dt
         self.substeps_num = substeps_num
         # Physics constants (xpbd_base.py:33-36)
-        self.gravity = _Scalar(_vec((0.0, -9.8, 0.0)), on_change=self._push_gravity)
+        self.gravity = _Scalar(_vec((0.0, -9.8, 0.0)), on_change=_weak_method(self._push_gravity))
         self.damping = 5.0
         self.total_mass = 1.0
         self.block_size = block_size
@@ -130,16 +160,18 @@ class XPBDSolverBase:
             self._rest,
             _Elements(
                 self.n_verts,
-                x=_VertexAttr(self, self._get_positions, self._set_positions),
-                v=_VertexAttr(self, self._get_velocities, self._set_velocities),
-                pred_x=_VertexAttr(self, self._get_predicted),
-                m=_VertexAttr(self, lambda: self._get_masses()[0], width=1),
-                inv_m=_VertexAttr(self, lambda: self._get_masses()[1], width=1),
+                x=_VertexAttr(self, lambda s: s._get_positions(),
+                              lambda s, a: s._set_positions(a)),
+                v=_VertexAttr(self, lambda s: s._get_velocities(),
+                              lambda s, a: s._set_velocities(a)),
+                pred_x=_VertexAttr(self, lambda s: s._get_predicted()),
+                m=_VertexAttr(self, lambda s: s._get_masses()[0], width=1),
+                inv_m=_VertexAttr(self, lambda s: s._get_masses()[1], width=1),
             ),
             _Elements(
                 self.n_tets,
-                lambda_D=_CellAttr(self, lambda: self._get_multipliers()[0]),
-                lambda_H=_CellAttr(self, lambda: self._get_multipliers()[1]),
+                lambda_D=_CellAttr(self, lambda s: s._get_multipliers()[0]),
+                lambda_H=_CellAttr(self, lambda s: s._get_multipliers()[1]),
             ),
         )
         self.indices = _IndexField(self)

@@ -195,3 +195,25 @@ def test_batch_follows_member_parameter_changes():
             a.advance(sub_dt, a.damping)
     for a, m in zip(alone, mixed):
         assert np.array_equal(a.mesh.verts.x.to_numpy(), m.mesh.verts.x.to_numpy())
+
+
+def test_dropped_solver_is_freed_at_once():
+    """The mesh views and the gravity callback hold the solver weakly, so a
+    dropped solver frees its device state when its last reference goes, not
+    whenever the cycle collector runs (a device free waits for the device)."""
+    import gc
+    import weakref
+    print("[case] dropped solver", flush=True)
+    gc.disable()
+    try:
+        s = make_solver(resting_cube(n=4, frames=1))
+        s.gravity[None] = (0.0, -5.0, 0.0)
+        s.solve()
+        x = s.mesh.verts.x
+        ref = weakref.ref(s)
+        del s
+        assert ref() is None
+        with pytest.raises(ReferenceError):
+            x.to_numpy()
+    finally:
+        gc.enable()
