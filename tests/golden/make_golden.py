@@ -12,7 +12,13 @@ mesh from there and the parameters of the reference's demo driver
 substep, 200 iterations, fixed plane y=-2 and fixed sphere r=0.3 at (0,-1,0),
 offset (0,1,0), gamma 0.666).  It starts from the state of that run after 54
 frames (as computed by the f32 oracle), just before the cow reaches the
-sphere, and covers the next 6 frames of contact.
+sphere, and covers the next 6 frames of contact.  The four `cow_compare_*`
+cases are the reference's convergence-comparison preset
+(scripts/compare_convergence_armadillo.bat: the same driver parameters with
+the convergence monitor on, one run per method: split without and with
+Chebyshev, block without and with Chebyshev) on the demo's cow from that same
+frame-54 state, 3 frames each; they take the mesh and starting state from the
+cow_demo_contact fixture instead of storing them again.
 
 Usage (from the repository root; reads the reference checkout only for the
 cow mesh):  python tests/golden/make_golden.py [--reference /root/reference]
@@ -118,19 +124,30 @@ def cases(reference: str):
         verts=cow.verts, tets=cow.tets, x0=warm.x.astype(np.float32),
         v0=warm.v.astype(np.float32), rho0=float(warm.rho_hat), sdfs=sdfs,
         gravity=(0.0, -9.8, 0.0), total_mass=1.0, frames=6, params=demo)
+    # the convergence-comparison preset's four methods from the same state
+    for tag, block, cheb in (("nh", False, False), ("nh_dc", False, True),
+                             ("block", True, False), ("block_dc", True, True)):
+        out["cow_compare_" + tag] = dict(
+            out["cow_demo_contact"], frames=3, base="cow_demo_contact",
+            params=dict(demo, block_neohookean_enable=block, cheb_enable=cheb,
+                        monitor_convergence=True))
     return out
 
 
 def write(name, case):
     x32, log32 = run_oracle(case, 32)
     x64, log64 = run_oracle(case, 64)
+    empty = dict(verts=np.zeros((0, 3), np.float32), tets=np.zeros((0, 4), np.int32))
+    based = case.get("base") is not None  # mesh and starting state stored in the base case
     arrays = dict(
-        verts=np.asarray(case["verts"], np.float32), tets=np.asarray(case["tets"], np.int32),
-        x0=np.asarray(case["x0"], np.float32) if case.get("x0") is not None else np.zeros((0, 3), np.float32),
-        v0=np.asarray(case["v0"], np.float32) if case.get("v0") is not None else np.zeros((0, 3), np.float32),
+        verts=empty["verts"] if based else np.asarray(case["verts"], np.float32),
+        tets=empty["tets"] if based else np.asarray(case["tets"], np.int32),
+        x0=np.asarray(case["x0"], np.float32) if case.get("x0") is not None and not based else np.zeros((0, 3), np.float32),
+        v0=np.asarray(case["v0"], np.float32) if case.get("v0") is not None and not based else np.zeros((0, 3), np.float32),
         meta=np.array(json.dumps(dict(
             sdfs=case["sdfs"], params=case["params"], gravity=list(case["gravity"]),
-            total_mass=case["total_mass"], frames=case["frames"], rho0=case.get("rho0")))),
+            total_mass=case["total_mass"], frames=case["frames"],
+            rho0=None if based else case.get("rho0"), base=case.get("base")))),
         x32=x32.astype(np.float32), x64=x64, log32=log32, log64=log64)
     path = os.path.join(HERE, name + ".npz")
     np.savez_compressed(path, **arrays)
@@ -140,9 +157,11 @@ def write(name, case):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only", nargs="*", default=None, help="write only these cases")
     args = ap.parse_args()
     for name, case in cases(args.reference).items():
-        write(name, case)
+        if args.only is None or name in args.only:
+            write(name, case)
 
 
 if __name__ == "__main__":

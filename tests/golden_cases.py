@@ -21,6 +21,11 @@ def load(name: str) -> dict:
     meta = json.loads(str(case.pop("meta")))
     case.update(meta)
     case["name"] = name
+    if case.get("base"):  # mesh and starting state kept in another fixture
+        base = load(case["base"])
+        for k in ("verts", "tets", "x0", "v0", "rho0"):
+            case[k] = base[k]
+        return case
     if case["x0"].size == 0:
         case["x0"] = None
     if case["v0"].size == 0:

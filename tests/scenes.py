@@ -66,16 +66,17 @@ def position_spread(x0, variants) -> float:
                default=0.0)
 
 
-def rhat_floor(log0, variants, agree=0.005):
-    """The f32 floor of a residual curve: the r_hat level above which every
-    rounding variant agrees with the default run within `agree` (relative);
-    iterations below it are round-off.  0.0 when they agree everywhere, None
-    when they disagree even at the largest r_hat."""
-    r0 = log0[:, 3]
+def rhat_floor(log0, variants, agree=0.005, col=3):
+    """The f32 floor of a logged curve (r_hat, column 3 of the oracle log, by
+    default; 6 and 7 are the monitor's mean |C_H| and |C_D|): the level above
+    which every rounding variant agrees with the default run within `agree`
+    (relative); iterations below it are round-off.  0.0 when they agree
+    everywhere, None when they disagree even at the largest value."""
+    r0 = log0[:, col]
     spread = np.zeros_like(r0)
     for v in variants:
         with np.errstate(divide="ignore", invalid="ignore"):
-            rel = np.where(r0 > 0, np.abs(v[1][:, 3] - r0) / r0, np.inf)
+            rel = np.where(r0 > 0, np.abs(v[1][:, col] - r0) / r0, np.inf)
         spread = np.maximum(spread, rel)
     order = np.argsort(-r0, kind="stable")
     worst = np.maximum.accumulate(spread[order])

@@ -33,7 +33,9 @@ def run(case, precision):
 def test_fixtures_present():
     assert set(golden_cases.names()) >= {"tet1_stretch", "tet2_fall", "box2_contact",
                                          "box4_split_monitor", "box4_split_cheb",
-                                         "cow_demo_contact"}
+                                         "cow_demo_contact", "cow_compare_nh",
+                                         "cow_compare_nh_dc", "cow_compare_block",
+                                         "cow_compare_block_dc"}
 
 
 @pytest.mark.parametrize("name", golden_cases.names())
