@@ -273,7 +273,7 @@ typedef struct xpbd_batch_opts {
                            * workgroups meeting at that member's own barriers):
                            * 0 never; 1 (default) when every member could take it alone,
                            * none spans more than 96 patches and the members hold at most
-                           * 256 patches in all (one workgroup per 2 patches); 2 whenever
+                           * 512 patches in all (one workgroup per 2 patches); 2 whenever
                            * every member could take it alone and they fit, else the
                            * substep fails with XPBD_EINVAL.  Other batches take the
                            * phased kernels (graph or launches as above). */

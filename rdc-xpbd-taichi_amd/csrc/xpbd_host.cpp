@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -442,6 +443,10 @@ struct xpbd_solver {
   // has run since the error word was last read
   DevBuf<unsigned> sync;
   int persist_capacity = 0;
+  // workgroups of that kernel resident at once on one XCD, and the XCD this
+  // solver's persistent substep runs on when the scene fits there
+  int persist_xcd_capacity = 0;
+  int home_xcd = 0;
   bool persist_pending = false;
   unsigned long long persist_timeout = kPersistTimeoutTicks;
   int n_patches = 0;
@@ -970,18 +975,18 @@ int fetch_log(xpbd_solver* s) {
 
 // The largest scene the persistent substep is used for.
 constexpr int kPersistMaxGroups = 128;  // 256 patches
-// XPBD_PATH_AUTO takes the persistent substep up to this many groups: a scene
-// barrier costs 1.6 us at 8 workgroups, 2.6 us at 24, 3.8 us at 48 (agent-scope
-// release and acquire per workgroup), and two per iteration beat the phased
-// launches up to about this size.
+// XPBD_PATH_AUTO takes the persistent substep up to this many groups.  Per
+// iteration on resting boxes (persistent against the graph of phased
+// kernels): 24 groups 10.8-11.7 against 16.0 us, 47 groups 14.3 against 16.1,
+// 81 groups 16.5 against 16.7, 116 groups 19.2 against 16.6.
 constexpr int kAutoPersistGroups = 48;
 // A batch takes the persistent substep automatically up to this many
-// workgroups in all.  Each workgroup's agent-scope release and acquire is an
-// L2 write-back or invalidate on its XCD, and with more workgroups these
-// queue up: for members of about 20 groups an iteration costs 11.5 us at 19
-// groups in all, 13.5 us at 80, 17.3 us at 160 and 20.4 us at 222, against
-// 15.6-16.8 us for the phased launches.
-constexpr int kAutoBatchGroups = 128;
+// workgroups in all (every member within kAutoPersistGroups).  With the scene
+// barriers free of L2 write-backs and invalidates, a batch of small members
+// stays near the one-member cost as it grows: members of 17 groups run an
+// iteration in 10.9 us at 34 groups in all, 11.2 us at 136 and 12.2 us at
+// 238, against 14.8-17.3 us for the phased launches.
+constexpr int kAutoBatchGroups = 256;
 
 // Whether the persistent substep kernel can run this scene: the fused
 // sequence with at least one iteration, and few enough groups to be resident
@@ -1000,8 +1005,13 @@ int run_substep(xpbd_solver* s, double sub_dt) {
                          s->host_scene.n_partials <= kAutoPersistGroups)) &&
                        !s->timing && persistent_ok(s);
   if (persist) {
-    HIP_TRY(xpbd::launch_substep_persistent(s->table.p, s->pitems.p, s->host_scene.n_partials,
-                                            s->persist_timeout, s->stream));
+    // all of the scene's workgroups on one XCD when they fit there: its
+    // barriers then meet in one L2
+    const int groups = s->host_scene.n_partials, nx = xcd_count();
+    const bool one_xcd = nx > 1 && groups <= s->persist_xcd_capacity;
+    HIP_TRY(xpbd::launch_substep_persistent(s->table.p, s->pitems.p, groups, s->persist_timeout,
+                                            one_xcd ? nx : 1, one_xcd ? s->home_xcd % nx : 0,
+                                            s->stream));
     s->persist_pending = true;
     return fetch_log(s);
   }
@@ -1283,6 +1293,9 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
     int occ_s = 0;
     HIP_TRY(xpbd::persistent_occupancy(&occ_s));
     s->persist_capacity = n_cu * occ_s;
+    s->persist_xcd_capacity = n_cu / xcd_count() * occ_s;
+    static std::atomic<unsigned> next_xcd{0};
+    s->home_xcd = int(next_xcd.fetch_add(1u) % unsigned(xcd_count()));
   }
   TRY(s->sync.alloc(4));
   TRY(s->sync.zero());
@@ -1787,9 +1800,11 @@ struct xpbd_batch {
   // launch's error word, and whether it has run since the word was last read
   DevBuf<PersistGroup> pgroups;
   PersistGroup* pgroups_d = nullptr;
-  int n_pgroups = 0;
+  int n_pgroups = 0;       // entries of the groups list: the launch's grid
+  int n_pgroups_live = 0;  // of which workgroups with work (the rest are holes)
   DevBuf<unsigned> perr;
   int persist_capacity = 0;
+  int persist_xcd_capacity = 0;
   bool persist_pending = false;
   unsigned long long persist_timeout = kPersistTimeoutTicks;
 
@@ -1885,6 +1900,36 @@ int batch_check_members(xpbd_solver* const* scenes, int n, int device) {
   return XPBD_OK;
 }
 
+// Places each member's persistent-substep workgroups on one XCD (dispatch is
+// round-robin over the n_xcd XCDs by block index, so entry slot * n_xcd + x
+// runs on XCD x): members in order, each to the XCD with the fewest
+// workgroups so far, so that a member's scene barriers meet in one L2.
+// Unused entries are holes (scene -1) that leave at once.  The list is left
+// as it is (contiguous, members spread over the XCDs) when some member would
+// not fit on one XCD beside those placed before it (`cap` workgroups
+// resident per XCD).
+int persist_xcd_layout(std::vector<PersistGroup>& pg, int n_xcd, int cap) {
+  if (pg.empty() || n_xcd <= 1 || cap < 1) return XPBD_OK;
+  std::vector<int> load(n_xcd, 0);
+  std::vector<int2> at(pg.size());  // (xcd, slot) of each entry
+  size_t i = 0;
+  while (i < pg.size()) {
+    size_t e = i;
+    while (e < pg.size() && pg[e].scene == pg[i].scene) ++e;
+    const int n = int(e - i);
+    const int x = int(std::min_element(load.begin(), load.end()) - load.begin());
+    if (load[x] + n > cap) return XPBD_OK;
+    for (int k = 0; k < n; ++k) at[i + k] = make_int2(x, load[x] + k);
+    load[x] += n;
+    i = e;
+  }
+  const int rows = *std::max_element(load.begin(), load.end());
+  std::vector<PersistGroup> out(size_t(rows) * n_xcd, PersistGroup{-1, 0, 0, 0});
+  for (size_t k = 0; k < pg.size(); ++k) out[size_t(at[k].y) * n_xcd + at[k].x] = pg[k];
+  pg.swap(out);
+  return XPBD_OK;
+}
+
 // The batch's concatenated work lists (vertex blocks and patch items in member
 // order), its persistent grids and ranges, and its scene table, for the given
 // members.  They go to the batch's own device buffers (reused when large
@@ -1910,6 +1955,8 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes, bool no
     n_tets += s->nt;
   }
   const int nx = o.xcd_split ? xcd_count() : 1;
+  const int n_pg_live = int(pg.size());
+  if (nx > 1) TRY(persist_xcd_layout(pg, nx, b->persist_xcd_capacity));
   std::vector<WorkItem> vmo(vm.size());
   const std::vector<int> order = xcd_order(int(vm.size()), nx);
   for (size_t k = 0; k < vm.size(); ++k) vmo[k] = vm[order[k]];
@@ -1974,6 +2021,7 @@ int batch_build(xpbd_batch* b, xpbd_solver* const* scenes, int n_scenes, bool no
     b->noalloc = false;
   }
   b->n_pgroups = int(pg.size());
+  b->n_pgroups_live = n_pg_live;
   b->scenes.assign(scenes, scenes + n_scenes);
   b->n_verts = n_verts;
   b->n_tets = n_tets;
@@ -2036,13 +2084,14 @@ void batch_count_substep(xpbd_batch* b) {
 // scene, that no member's barrier spans more than kAutoPersistGroups
 // workgroups, and that the launch stays within kAutoBatchGroups.
 bool batch_persistent(const xpbd_batch* b) {
-  if (b->opts.persistent == 0 || b->n_pgroups < 1 || b->n_pgroups > b->persist_capacity)
+  if (b->opts.persistent == 0 || b->n_pgroups_live < 1 ||
+      b->n_pgroups_live > b->persist_capacity)
     return false;
   for (const xpbd_solver* s : b->scenes) {
     if (!persistent_ok(s)) return false;
     if (b->opts.persistent == 1 && s->host_scene.n_partials > kAutoPersistGroups) return false;
   }
-  return b->opts.persistent == 2 || b->n_pgroups <= kAutoBatchGroups;
+  return b->opts.persistent == 2 || b->n_pgroups_live <= kAutoBatchGroups;
 }
 
 bool batch_at_frame_start(const xpbd_batch* b) {
@@ -2153,6 +2202,7 @@ int xpbd_batch_create(xpbd_solver* const* scenes, int32_t n_scenes, const xpbd_b
     HIP_TRY(xpbd::persistent_occupancy(&occ));
     HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, b->device));
     b->persist_capacity = n_cu * occ;
+    b->persist_xcd_capacity = n_cu / xcd_count() * occ;
   }
   TRY(b->perr.alloc(1));
   TRY(b->perr.zero());

@@ -36,10 +36,14 @@ hipError_t launch_advance(const SceneDev* scenes, const WorkItem* vmap, int nblo
                           hipStream_t st);
 // One substep of one scene in one launch (n_groups workgroups, all resident);
 // a scene barrier gives up after timeout_ticks of the 100 MHz wall clock.
+// xcd_stride > 1 places every workgroup on XCD `xcd` (grid n_groups *
+// xcd_stride, round-robin dispatch; the other blocks leave at once).
 hipError_t launch_substep_persistent(const SceneDev* scenes, const PatchItem* items, int n_groups,
-                                    unsigned long long timeout_ticks, hipStream_t st);
+                                    unsigned long long timeout_ticks, int xcd_stride, int xcd,
+                                    hipStream_t st);
 // One substep of every member of a batch in one launch: workgroup b does
-// groups[b] (n_groups in all, all resident); a barrier that gives up sets *err.
+// groups[b] (n_groups in all, holes marked by scene -1 leave at once; the
+// others all resident); a barrier that gives up sets *err.
 hipError_t launch_batch_substep_persistent(const SceneDev* scenes, const PatchItem* items,
                                           const PersistGroup* groups, int n_groups,
                                           unsigned* err, unsigned long long timeout_ticks,

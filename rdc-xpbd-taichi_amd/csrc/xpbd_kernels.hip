@@ -190,6 +190,37 @@ __device__ __forceinline__ void block_sums(float r, float ah, float ad, bool all
   }
 }
 
+// Data that the workgroups of one persistent launch hand to each other across
+// a scene barrier (slot partials, residual partials) is stored and loaded as
+// agent-scope atomics: global stores and loads with the sc1 bit, performed at
+// the device's point of coherence, so they need no L2 write-back before the
+// barrier and no L2 invalidate after it.  A float4 goes as two 8-byte words.
+__device__ __forceinline__ void xstore4(gptr<float4> p, float4 v) {
+  gptr<unsigned long long> q = (gptr<unsigned long long>)p;
+  __hip_atomic_store(q, __builtin_bit_cast(unsigned long long, make_float2(v.x, v.y)),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, __builtin_bit_cast(unsigned long long, make_float2(v.z, v.w)),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float4 xload4(gptr<const float4> p) {
+  gptr<const unsigned long long> q = (gptr<const unsigned long long>)p;
+  const float2 a = __builtin_bit_cast(
+      float2, __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  const float2 b = __builtin_bit_cast(
+      float2, __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+  return make_float4(a.x, a.y, b.x, b.y);
+}
+__device__ __forceinline__ void xstore_d(gptr<double> p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double xload_d(gptr<const double> p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// s_waitcnt immediate that waits for vector memory only (gfx9 encoding:
+// vmcnt 0 in bits 3:0 and 15:14, expcnt 7 in bits 6:4, lgkmcnt 15 in 11:8).
+constexpr unsigned kWaitVmcnt0 = 0x0F70;
+
 // Stores partial idx (component c at partial[c * n_partials + idx]).
 __device__ __forceinline__ void store_partial(const SceneDev& S, int idx, const double tot[3]) {
   const size_t n = size_t(S.n_partials);
@@ -198,22 +229,37 @@ __device__ __forceinline__ void store_partial(const SceneDev& S, int idx, const 
   S.partial[2 * n + idx] = tot[2];
 }
 
+// store_partial for partials read by other workgroups of the same launch.
+__device__ __forceinline__ void store_partial_x(const SceneDev& S, int idx, const double tot[3]) {
+  const size_t n = size_t(S.n_partials);
+  xstore_d(S.partial + idx, tot[0]);
+  xstore_d(S.partial + n + idx, tot[1]);
+  xstore_d(S.partial + 2 * n + idx, tot[2]);
+}
+
 // delta_x (xyz) and the constraint count (w) of vertex v summed over its slots
-// [b, e) in ascending patch order.
+// [b, e) in ascending patch order.  kX: the slots were written by other
+// workgroups of the same launch (exchange loads).
+template <bool kX = false>
 __device__ __forceinline__ float4 slot_sum(const SceneDev& S, int b, int e) {
+  auto ld = [&](int i) -> float4 {
+    if constexpr (kX)
+      return xload4(S.slot_acc + i);
+    else
+      return S.slot_acc[i];
+  };
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
   int a = b;
   // four slot loads in flight, added in the same order as one at a time
   for (; a + 4 <= e; a += 4) {
-    const float4 c0 = S.slot_acc[a], c1 = S.slot_acc[a + 1], c2 = S.slot_acc[a + 2],
-                 c3 = S.slot_acc[a + 3];
+    const float4 c0 = ld(a), c1 = ld(a + 1), c2 = ld(a + 2), c3 = ld(a + 3);
     acc.x += c0.x; acc.y += c0.y; acc.z += c0.z; acc.w += c0.w;
     acc.x += c1.x; acc.y += c1.y; acc.z += c1.z; acc.w += c1.w;
     acc.x += c2.x; acc.y += c2.y; acc.z += c2.z; acc.w += c2.w;
     acc.x += c3.x; acc.y += c3.y; acc.z += c3.z; acc.w += c3.w;
   }
   for (; a < e; ++a) {
-    const float4 c = S.slot_acc[a];
+    const float4 c = ld(a);
     acc.x += c.x;
     acc.y += c.y;
     acc.z += c.z;
@@ -954,23 +1000,26 @@ __global__ void k_advance(const SceneDev* __restrict__ scenes, const WorkItem* _
 // ---- persistent substep (small scenes) -------------------------------------------
 
 // A scene-wide barrier over the persistent substep's workgroups of one scene.
-// The workgroup barrier orders every thread's global writes before thread 0
-// (work-group scope), which releases them at agent scope, counts the
-// workgroup in on the scene's arrival word and waits, without blocking
-// forever, until `target` arrivals are counted; it then acquires at agent
-// scope, and the closing workgroup barrier passes that on to the other
-// threads.  (One release and one acquire per workgroup: each is an L2
-// write-back or invalidate on this GPU.)  The wait ends early when any
-// workgroup has set the error word, and after `timeout` ticks of the 100 MHz
-// wall clock it sets the error word itself (the workgroups are not all
-// resident).  Returns false on either: the caller then leaves its loops.
+// Everything the workgroups hand to each other across it is written and read
+// with agent-scope atomics (xstore4 / xstore_d, xload4 / xload_d), so the
+// barrier orders only their completion: every wave first waits until its own
+// stores have completed (vmcnt 0), the workgroup barrier then holds thread 0
+// until all waves have, and thread 0 counts the workgroup in on the scene's
+// arrival word and waits, without blocking forever, until `target` arrivals
+// are counted.  Loads issued after the closing workgroup barrier see every
+// counted workgroup's stores.  No L2 write-back or invalidate is issued (an
+// agent-scope release and acquire would be one of each, per workgroup, on
+// the XCD that runs it).  The wait ends early when any workgroup has set the
+// error word, and after `timeout` ticks of the 100 MHz wall clock it sets the
+// error word itself (the workgroups are not all resident).  Returns false on
+// either: the caller then leaves its loops.
 __device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, gptr<unsigned> err,
                                               unsigned target, unsigned long long timeout,
                                               int* s_ok) {
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __hip_atomic_fetch_add(&sync[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned long long t0 = wall_clock64();
     while (__hip_atomic_load(&sync[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -985,7 +1034,6 @@ __device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, gptr<unsigned
       }
       __builtin_amdgcn_s_sleep(1);
     }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     *s_ok = ok;
   }
   __syncthreads();
@@ -1009,7 +1057,7 @@ __device__ __forceinline__ void scene_sums_first(const SceneDev& S, bool all3, d
       for (int b = tid; b < n; b += 8 * kReduceThreads) {
         double v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = p[min(b + k * kReduceThreads, n - 1)];
+        for (int k = 0; k < 8; ++k) v[k] = xload_d(p + min(b + k * kReduceThreads, n - 1));
 #pragma unroll
         for (int k = 0; k < 8; ++k)
           if (b + k * kReduceThreads < n) sum += v[k];
@@ -1060,13 +1108,21 @@ template <bool kBatch>
 __global__ void __launch_bounds__(kPersistThreads)
     k_substep_persistent(const SceneDev* __restrict__ scenes, const PatchItem* __restrict__ items,
                          const PersistGroup* __restrict__ groups, unsigned* err_word,
-                         unsigned long long timeout) {
+                         unsigned long long timeout, int xcd_stride, int xcd) {
   int scene = 0, group = blockIdx.x;
   if constexpr (kBatch) {
     const PersistGroup d = groups[blockIdx.x];
+    if (d.scene < 0) return;  // a hole in the per-XCD layout
     scene = d.scene;
     group = d.group;
     items += d.item0;
+  } else {
+    // Workgroups are dispatched to the XCDs round-robin by block index: with
+    // xcd_stride = the XCD count, the scene's group g is block
+    // g * xcd_stride + xcd, so all of its workgroups share one L2.  The other
+    // blocks of the grid (xcd_stride * groups) leave at once.
+    if (int(blockIdx.x) % xcd_stride != xcd) return;
+    group = int(blockIdx.x) / xcd_stride;
   }
   const SceneDev& S = scenes[scene];
   gptr<unsigned> err;
@@ -1218,12 +1274,12 @@ __global__ void __launch_bounds__(kPersistThreads)
       } else {
         a = my_sorted[lt];
       }
-      S.slot_acc[vs.y] = a;
+      xstore4(S.slot_acc + vs.y, a);
     }
     ok = scene_barrier(sync, err, ++barriers * G, timeout, &s_ok);
     if (!ok) break;
     // Jacobi average of the patch's vertices from every slot of each
-    if (lt < h.n_verts) s_x[half][lt] = jacobi_from(S, slot_sum(S, sb, se), val, s_y[half][lt]);
+    if (lt < h.n_verts) s_x[half][lt] = jacobi_from(S, slot_sum<true>(S, sb, se), val, s_y[half][lt]);
     __syncthreads();
     // residual at x_J: lane sums over the group's two patches in order
     float rv[kPatchTetsPerThread], hv[kPatchTetsPerThread], dv[kPatchTetsPerThread];
@@ -1266,7 +1322,7 @@ __global__ void __launch_bounds__(kPersistThreads)
     }
     double tot[3];
     block_sums(acc_r, acc_h, acc_d, S.monitor != 0, tot);
-    if (tid == 0) store_partial(S, group, tot);
+    if (tid == 0) store_partial_x(S, group, tot);
     ok = scene_barrier(sync, err, ++barriers * G, timeout, &s_ok);
   }
   if (ok) {
@@ -1455,10 +1511,12 @@ hipError_t launch_final_blend(const SceneDev* scenes, const WorkItem* vmap, int 
   return hipGetLastError();
 }
 hipError_t launch_substep_persistent(const SceneDev* scenes, const PatchItem* items, int n_groups,
-                                    unsigned long long timeout_ticks, hipStream_t st) {
-  if (n_groups <= 0) return hipErrorInvalidValue;
-  k_substep_persistent<false><<<n_groups, kPersistThreads, 0, st>>>(scenes, items, nullptr,
-                                                                    nullptr, timeout_ticks);
+                                    unsigned long long timeout_ticks, int xcd_stride, int xcd,
+                                    hipStream_t st) {
+  if (n_groups <= 0 || xcd_stride < 1 || xcd < 0 || xcd >= xcd_stride)
+    return hipErrorInvalidValue;
+  k_substep_persistent<false><<<n_groups * xcd_stride, kPersistThreads, 0, st>>>(
+      scenes, items, nullptr, nullptr, timeout_ticks, xcd_stride, xcd);
   return hipGetLastError();
 }
 hipError_t launch_batch_substep_persistent(const SceneDev* scenes, const PatchItem* items,
@@ -1467,7 +1525,7 @@ hipError_t launch_batch_substep_persistent(const SceneDev* scenes, const PatchIt
                                           hipStream_t st) {
   if (n_groups <= 0 || !groups || !err) return hipErrorInvalidValue;
   k_substep_persistent<true><<<n_groups, kPersistThreads, 0, st>>>(scenes, items, groups, err,
-                                                                   timeout_ticks);
+                                                                   timeout_ticks, 1, 0);
   return hipGetLastError();
 }
 hipError_t persistent_occupancy(int* blocks_per_cu) {

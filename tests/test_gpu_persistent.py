@@ -155,6 +155,46 @@ def test_batch_persistent_matches_phased_and_alone(persistent):
             assert s.current_frame == 2
 
 
+def test_persistent_placement_keeps_the_bits(monkeypatch):
+    """The single-scene persistent substep places its workgroups on one XCD
+    when they fit there (24 for the cube); spread over all XCDs (one XCD
+    declared) the results are the same bits."""
+    print("[case] persistent placement", flush=True)
+    scene = resting_cube(frames=2)
+    one = make_solver(scene, log_residuals=True, exec_path="persistent")
+    monkeypatch.setenv("XPBD_XCDS", "1")
+    spread = make_solver(scene, log_residuals=True, exec_path="persistent")
+    for s in (one, spread):
+        for _ in range(scene.frames):
+            s.solve()
+    _same(one, spread)
+
+
+@pytest.mark.parametrize("n_members,xcd_split", [(4, True), (4, False), (10, True)])
+def test_batch_persistent_layouts_keep_the_bits(n_members, xcd_split):
+    """Members of 17 groups each: four fit one XCD apiece (per-XCD layout with
+    holes), and without the XCD split they run contiguously; ten do not fit
+    (two on one XCD would need 34 of its 32 CUs), so the layout stays
+    contiguous.  Every member matches the scene run alone, bit for bit."""
+    print(f"[case] batch persistent layout {n_members} {xcd_split}", flush=True)
+    scene = resting_cube(n=14, frames=2)
+    scene.params.update(substeps_num=2, constraints_iter=12)
+    alone = make_solver(scene, exec_path="graph")
+    for _ in range(2):
+        alone.solve()
+    members = [make_solver(scene) for _ in range(n_members)]
+    with SceneBatch(members, persistent="always", xcd_split=xcd_split) as b:
+        assert b.path == "persistent"
+        b.frame(2)
+        b.synchronize()
+    for s in members:
+        assert np.array_equal(alone.mesh.verts.x.to_numpy(), s.mesh.verts.x.to_numpy())
+        assert np.array_equal(alone.mesh.verts.v.to_numpy(), s.mesh.verts.v.to_numpy())
+        assert np.array_equal(alone.mesh.cells.lambda_H.to_numpy(),
+                              s.mesh.cells.lambda_H.to_numpy())
+        assert alone.rho_hat == s.rho_hat
+
+
 def test_batch_persistent_path_selection():
     print("[case] batch persistent path selection", flush=True)
     small = [make_solver(s) for s in batched_scenes(n_scenes=3, seed=5, n_range=(3, 6), iters=5)]
