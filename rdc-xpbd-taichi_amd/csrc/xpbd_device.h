@@ -96,8 +96,11 @@ struct SceneDev {
                          // component c at [c * n_partials])
   gptr<SceneCtrl> ctrl;   // two slots, see the controller update in the kernels
   gptr<double> log;      // 6 doubles per iteration: iter, r_hat, rho_hat, omega, hydro, dev
-  // persistent substep: grid-barrier arrivals, workgroups done, error word
+  // persistent substep: scene-barrier arrivals, workgroups done, error word
   gptr<unsigned> sync;
+  // persistent substep: residual partials posted by value, kMailboxWords per
+  // residual group (2 iterations' parity x 3 components, SoA over groups)
+  gptr<double> mailbox;
   int n_verts;
   int n_tets;
   int n_partials;
@@ -163,6 +166,15 @@ constexpr int kPartialGroup = 2;
 // Persistent substep kernel: one workgroup of kPersistThreads threads per
 // residual group of kPartialGroup patches (one patch per half).
 constexpr int kPersistThreads = kPatchThreads * kPartialGroup;
+
+// The persistent substep's residual mailbox: per residual group, 2
+// iterations' parity x 3 components, each a double posted by value.  A word
+// holding kMailboxEmpty has not been posted: no double widened from f32 or
+// summed from such doubles has this bit pattern (a NaN with all 52 payload
+// bits set; widened f32 NaNs keep the low 29 bits zero, and arithmetic
+// propagates payloads without setting bits).
+constexpr int kMailboxWords = 6;
+constexpr unsigned long long kMailboxEmpty = ~0ull;
 
 // One workgroup of the batched persistent substep: the member scene, its
 // residual group in that scene, and where the scene's patch items start in

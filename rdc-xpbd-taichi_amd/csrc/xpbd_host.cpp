@@ -442,6 +442,9 @@ struct xpbd_solver {
   // most workgroups of its kernel that can be resident at once, and whether it
   // has run since the error word was last read
   DevBuf<unsigned> sync;
+  // persistent substep: each residual group's partials of the last two
+  // iterations, posted by value (empty words hold kMailboxEmpty)
+  DevBuf<double> mailbox;
   int persist_capacity = 0;
   // workgroups of that kernel resident at once on one XCD, and the XCD this
   // solver's persistent substep runs on when the scene fits there
@@ -496,7 +499,7 @@ struct xpbd_solver {
     add(vslot_off); add(p_vert_sum); add(tet_local); add(pitems); add(prange_proj);
     add(prange_res); add(valence); add(adj_off); add(adj); add(d_old2new); add(tet); add(dm0);
     add(dm1); add(dm2); add(lam); add(partial); add(logbuf); add(ctrl); add(table); add(vmap);
-    add(tmap); add(scratch_xyz); add(flag); add(lam_save); add(sync);
+    add(tmap); add(scratch_xyz); add(flag); add(lam_save); add(sync); add(mailbox);
     return b;
   }
 
@@ -613,6 +616,7 @@ int build_scene(xpbd_solver* s) {
   S.ctrl = s->ctrl.p;
   S.log = s->logbuf.p;
   S.sync = s->sync.p;
+  S.mailbox = s->mailbox.p;
   S.n_verts = s->nv;
   S.n_tets = s->nt;
   S.layout = s->layout;
@@ -975,6 +979,8 @@ int fetch_log(xpbd_solver* s) {
 
 // The largest scene the persistent substep is used for.
 constexpr int kPersistMaxGroups = 128;  // 256 patches
+// (its residual mailbox is read by one thread per group of a 256-thread half)
+static_assert(kPersistMaxGroups <= xpbd::kPatchThreads, "mailbox readers");
 // XPBD_PATH_AUTO takes the persistent substep up to this many groups.  Per
 // iteration on resting boxes (persistent against the graph of phased
 // kernels): 24 groups 10.8-11.7 against 16.0 us, 47 groups 14.3 against 16.1,
@@ -1299,6 +1305,13 @@ int xpbd_create(const xpbd_desc* d, xpbd_solver** out) {
   }
   TRY(s->sync.alloc(4));
   TRY(s->sync.zero());
+  if (s->layout == XPBD_LAYOUT_PATCH) {
+    const size_t groups = size_t(s->n_patches + xpbd::kPartialGroup - 1) / xpbd::kPartialGroup;
+    if (groups <= size_t(kPersistMaxGroups)) {  // scenes the persistent substep can run
+      TRY(s->mailbox.alloc(xpbd::kMailboxWords * groups));
+      HIP_TRY(hipMemset(s->mailbox.p, 0xff, s->mailbox.n * sizeof(double)));  // all empty
+    }
+  }
   TRY(s->lam.alloc(nt));
   TRY(s->lam.zero());
   TRY(s->d_old2new.upload(s->v_old2new.data(), nv));

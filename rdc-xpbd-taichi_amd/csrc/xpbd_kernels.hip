@@ -190,11 +190,12 @@ __device__ __forceinline__ void block_sums(float r, float ah, float ad, bool all
   }
 }
 
-// Data that the workgroups of one persistent launch hand to each other across
-// a scene barrier (slot partials, residual partials) is stored and loaded as
-// agent-scope atomics: global stores and loads with the sc1 bit, performed at
-// the device's point of coherence, so they need no L2 write-back before the
-// barrier and no L2 invalidate after it.  A float4 goes as two 8-byte words.
+// Data that the workgroups of one persistent launch hand to each other (slot
+// partials across a scene barrier, residual partials through the mailbox) is
+// stored and loaded as agent-scope atomics: global stores and loads with the
+// sc1 bit, performed at the device's point of coherence, so they need no L2
+// write-back before the barrier and no L2 invalidate after it.  A float4 goes
+// as two 8-byte words.
 __device__ __forceinline__ void xstore4(gptr<float4> p, float4 v) {
   gptr<unsigned long long> q = (gptr<unsigned long long>)p;
   __hip_atomic_store(q, __builtin_bit_cast(unsigned long long, make_float2(v.x, v.y)),
@@ -213,9 +214,6 @@ __device__ __forceinline__ float4 xload4(gptr<const float4> p) {
 __device__ __forceinline__ void xstore_d(gptr<double> p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ double xload_d(gptr<const double> p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // s_waitcnt immediate that waits for vector memory only (gfx9 encoding:
 // vmcnt 0 in bits 3:0 and 15:14, expcnt 7 in bits 6:4, lgkmcnt 15 in 11:8).
@@ -229,12 +227,16 @@ __device__ __forceinline__ void store_partial(const SceneDev& S, int idx, const 
   S.partial[2 * n + idx] = tot[2];
 }
 
-// store_partial for partials read by other workgroups of the same launch.
-__device__ __forceinline__ void store_partial_x(const SceneDev& S, int idx, const double tot[3]) {
-  const size_t n = size_t(S.n_partials);
-  xstore_d(S.partial + idx, tot[0]);
-  xstore_d(S.partial + n + idx, tot[1]);
-  xstore_d(S.partial + 2 * n + idx, tot[2]);
+// Posts residual group `group`'s partials of iteration `it` to the scene's
+// mailbox (parity it & 1), each component an agent-scope atomic 8-byte store:
+// the value itself tells a reader that it has arrived.
+__device__ __forceinline__ void post_partials(const SceneDev& S, int it, int group,
+                                              const double tot[3]) {
+  const size_t G = size_t(S.n_partials);
+  gptr<double> box = S.mailbox + size_t(it & 1) * 3 * G + group;
+  xstore_d(box, tot[0]);
+  xstore_d(box + G, tot[1]);
+  xstore_d(box + 2 * G, tot[2]);
 }
 
 // delta_x (xyz) and the constraint count (w) of vertex v summed over its slots
@@ -1001,7 +1003,7 @@ __global__ void k_advance(const SceneDev* __restrict__ scenes, const WorkItem* _
 
 // A scene-wide barrier over the persistent substep's workgroups of one scene.
 // Everything the workgroups hand to each other across it is written and read
-// with agent-scope atomics (xstore4 / xstore_d, xload4 / xload_d), so the
+// with agent-scope atomics (xstore4, xload4), so the
 // barrier orders only their completion: every wave first waits until its own
 // stores have completed (vmcnt 0), the workgroup barrier then holds thread 0
 // until all waves have, and thread 0 counts the workgroup in on the scene's
@@ -1040,38 +1042,70 @@ __device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, gptr<unsigned
   return *s_ok != 0;
 }
 
-// scene_sums by the first kReduceThreads threads of a larger workgroup (the
-// others only meet the barriers), in the same order, so the same bits.
-__device__ __forceinline__ void scene_sums_first(const SceneDev& S, bool all3, double tot[3],
-                                                 double* red) {
+// The sums of every residual group's partials of iteration `it` (component 0,
+// and 1-2 when `all3`), waiting for each to be posted in the mailbox.  Thread
+// t < n_partials (at most kReduceThreads) polls group t's words with
+// agent-scope atomic loads until none is empty, so the values arrive with the
+// wait; the sums are then taken exactly as scene_sums takes them (thread t
+// holds entry t, then the wave sums, then the waves in order), so the same
+// bits.  A wave gives up when the error word is set, or sets it after
+// `timeout` wall-clock ticks.  Returns false on either (to every thread).
+// Thread 0 gets the totals.  Ends with a workgroup barrier.
+__device__ __forceinline__ bool mailbox_sums(const SceneDev& S, int it, bool all3, double tot[3],
+                                             double* red, gptr<unsigned> err,
+                                             unsigned long long timeout, int* s_wok) {
   constexpr int kW = kReduceThreads / 64;
-  const int n = S.n_partials;
-  const int tid = threadIdx.x;
+  const int G = S.n_partials;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   double acc[3] = {0.0, 0.0, 0.0};
   if (tid < kReduceThreads) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if (c > 0 && !all3) break;
-      gptr<const double> p = S.partial + size_t(c) * size_t(n);
-      double sum = 0.0;
-      for (int b = tid; b < n; b += 8 * kReduceThreads) {
-        double v[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = xload_d(p + min(b + k * kReduceThreads, n - 1));
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-          if (b + k * kReduceThreads < n) sum += v[k];
+    int ok = 1;
+    if (wave * 64 < G) {
+      gptr<const unsigned long long> box =
+          (gptr<const unsigned long long>)(S.mailbox + size_t(it & 1) * 3 * size_t(G));
+      const int nc = all3 ? 3 : 1;
+      unsigned long long u[3] = {0ull, 0ull, 0ull};
+      const unsigned long long t0 = wall_clock64();
+      while (true) {
+        bool have = true;
+        if (tid < G) {
+          for (int c = 0; c < nc; ++c) {
+            u[c] = __hip_atomic_load(box + size_t(c) * G + tid, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+            have = have && u[c] != kMailboxEmpty;
+          }
+        }
+        if (__all(have)) break;
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+          ok = 0;
+          break;
+        }
+        if (wall_clock64() - t0 > timeout) {
+          if (lane == 0) __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ok = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
       }
-      acc[c] = wave_sum_d(sum);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        if (c > 0 && !all3) break;
+        double sum = 0.0;
+        if (tid < G) sum += __builtin_bit_cast(double, u[c]);
+        acc[c] = wave_sum_d(sum);
+      }
     }
-    const int lane = tid & 63, wave = tid >> 6;
     if (lane == 0) {
+      s_wok[wave] = ok;
       red[wave] = acc[0];
       red[kW + wave] = acc[1];
       red[2 * kW + wave] = acc[2];
     }
   }
   __syncthreads();
+  bool all_ok = true;
+#pragma unroll
+  for (int w = 0; w < kW; ++w) all_ok = all_ok && s_wok[w] != 0;
   if (tid == 0) {
     tot[0] = tot[1] = tot[2] = 0.0;
     for (int w = 0; w < kW; ++w) {
@@ -1081,6 +1115,7 @@ __device__ __forceinline__ void scene_sums_first(const SceneDev& S, bool all3, d
     }
   }
   __syncthreads();
+  return all_ok;
 }
 
 // One whole substep of one scene (predict, every Jacobi iteration with its
@@ -1092,10 +1127,11 @@ __device__ __forceinline__ void scene_sums_first(const SceneDev& S, bool all3, d
 // the rest positions of the group's vertices stay in LDS, and the
 // multipliers in registers, for the whole substep; each patch predicts and
 // blends its own copies of its vertices (the same bits in every patch that
-// holds a vertex, as in the phased kernels).  Two scene barriers per
-// iteration: after the slot partials are written, and after the residual
-// partials.  Every sum is taken in the phased kernels' order, so the results
-// are bit-identical to them in deterministic mode.  Requires the fused
+// holds a vertex, as in the phased kernels).  Per iteration, one scene
+// barrier after the slot partials are written, and one wait for the residual
+// partials, which arrive by value in the scene's mailbox.  Every sum is taken
+// in the phased kernels' order, so the results are bit-identical to them in
+// deterministic mode.  Requires the fused
 // sequence (patch layout, Chebyshev on, no isolated vertices), at least one
 // iteration, and all of the launch's workgroups resident at once.
 //
@@ -1144,6 +1180,7 @@ __global__ void __launch_bounds__(kPersistThreads)
   __shared__ double s_red[kReduceScratch];
   __shared__ float s_omega;
   __shared__ int s_ok;
+  __shared__ int s_wok[kReduceThreads / 64];
   __shared__ SceneCtrl s_ctrl;
   gptr<unsigned> sync = S.sync;
   float4* my_sorted = s_sorted[half];
@@ -1197,9 +1234,11 @@ __global__ void __launch_bounds__(kPersistThreads)
   const bool writer = group == 0;
   for (int it = 0; it < S.iters && ok; ++it) {
     if (it > 0) {
-      // rho/omega update after iteration it - 1 from every group's partial
+      // rho/omega update after iteration it - 1 from every group's partial,
+      // as each arrives in the mailbox
       double tot[3];
-      scene_sums_first(S, S.monitor != 0, tot, s_red);
+      ok = mailbox_sums(S, it - 1, S.monitor != 0, tot, s_red, err, timeout, s_wok);
+      if (!ok) break;
       if (tid == 0) {
         double L[6];
         const SceneCtrl c = ctrl_update(S, s_ctrl, it - 1, tot, L);
@@ -1278,6 +1317,12 @@ __global__ void __launch_bounds__(kPersistThreads)
     }
     ok = scene_barrier(sync, err, ++barriers * G, timeout, &s_ok);
     if (!ok) break;
+    // Every group has taken the partials of iteration it - 1 before arriving
+    // here, so this group's words of that parity are emptied for it + 1.
+    if (tid < 3)
+      __hip_atomic_store((gptr<unsigned long long>)S.mailbox +
+                             (size_t((it + 1) & 1) * 3 + tid) * G + group,
+                         kMailboxEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // Jacobi average of the patch's vertices from every slot of each
     if (lt < h.n_verts) s_x[half][lt] = jacobi_from(S, slot_sum<true>(S, sb, se), val, s_y[half][lt]);
     __syncthreads();
@@ -1322,14 +1367,13 @@ __global__ void __launch_bounds__(kPersistThreads)
     }
     double tot[3];
     block_sums(acc_r, acc_h, acc_d, S.monitor != 0, tot);
-    if (tid == 0) store_partial_x(S, group, tot);
-    ok = scene_barrier(sync, err, ++barriers * G, timeout, &s_ok);
+    if (tid == 0) post_partials(S, it, group, tot);
   }
+  double tot[3];
+  if (ok) ok = mailbox_sums(S, S.iters - 1, true, tot, s_red, err, timeout, s_wok);
   if (ok) {
     // the update after the last iteration (k_reduce), the final blend
     // (k_final_blend) and advance (k_advance)
-    double tot[3];
-    scene_sums_first(S, true, tot, s_red);
     if (tid == 0) {
       double L[6];
       const SceneCtrl c = ctrl_update(S, s_ctrl, S.iters - 1, tot, L);
@@ -1351,13 +1395,21 @@ __global__ void __launch_bounds__(kPersistThreads)
     for (int j = 0; j < kPatchTetsPerThread; ++j)
       if (lt + j * kPatchThreads < h.n_tets) S.lam[h.tet_begin + lt + j * kPatchThreads] = t.lam[j];
   }
-  // the last workgroup out resets the barrier counters for the next launch
+  // the last workgroup out (every other has finished with the barrier words
+  // and the mailbox) resets them for the next launch
   __syncthreads();
   if (tid == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned done =
         __hip_atomic_fetch_add(&sync[1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (done == G - 1) {
+    s_ok = done == G - 1;
+  }
+  __syncthreads();
+  if (s_ok) {
+    gptr<unsigned long long> box = (gptr<unsigned long long>)S.mailbox;
+    for (int i = tid; i < kMailboxWords * int(G); i += kPersistThreads)
+      __hip_atomic_store(box + i, kMailboxEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
       __hip_atomic_store(&sync[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_store(&sync[1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
