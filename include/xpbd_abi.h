@@ -205,11 +205,11 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets);
 /* Execution-path selection (no reference counterpart).
  *  XPBD_PATH_KERNELS:    one launch per kernel per iteration.
  *  XPBD_PATH_GRAPH:      the same kernels, each substep captured in a hipGraph.
- *  XPBD_PATH_PERSISTENT: for small scenes (at most 256 patches of 512 tets,
+ *  XPBD_PATH_PERSISTENT: for small scenes (at most 512 patches of 512 tets,
  *                        the patch layout with Chebyshev on and no isolated
  *                        vertices, at least one iteration), one launch per
  *                        substep that runs every iteration with scene-wide
- *                        barriers between its phases; bit-identical to the
+ *                        waits between its phases; bit-identical to the
  *                        other paths in deterministic mode.  Other scenes take
  *                        the graph path.  If a barrier cannot complete (the
  *                        kernel's workgroups not all resident, e.g. beside
@@ -218,9 +218,9 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets);
 #define XPBD_PATH_KERNELS 0
 #define XPBD_PATH_GRAPH 1
 #define XPBD_PATH_PERSISTENT 2
-/*  XPBD_PATH_AUTO:       the persistent substep for scenes of at most 96
- *                        patches (where its scene barriers cost less than the
- *                        phased launches' dependent loads), the graph path
+/*  XPBD_PATH_AUTO:       the persistent substep for scenes of at most 288
+ *                        patches (where its waits cost less than the phased
+ *                        launches' dependent loads), the graph path
  *                        otherwise. */
 #define XPBD_PATH_AUTO 3
 int xpbd_set_exec_path(xpbd_solver* s, int32_t path);
@@ -272,8 +272,8 @@ typedef struct xpbd_batch_opts {
   int32_t persistent;     /* the persistent substep (one launch per substep, each member's
                            * workgroups meeting at that member's own barriers):
                            * 0 never; 1 (default) when every member could take it alone,
-                           * none spans more than 96 patches and the members hold at most
-                           * 512 patches in all (one workgroup per 2 patches); 2 whenever
+                           * none spans more than 288 patches and the members hold at
+                           * most 512 patches in all (one workgroup per 2 patches); 2 whenever
                            * every member could take it alone and they fit, else the
                            * substep fails with XPBD_EINVAL.  Other batches take the
                            * phased kernels (graph or launches as above). */

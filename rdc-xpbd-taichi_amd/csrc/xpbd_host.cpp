@@ -978,14 +978,16 @@ int fetch_log(xpbd_solver* s) {
 }
 
 // The largest scene the persistent substep is used for.
-constexpr int kPersistMaxGroups = 128;  // 256 patches
+constexpr int kPersistMaxGroups = 256;  // 512 patches, one workgroup per CU
 // (its residual mailbox is read by one thread per group of a 256-thread half)
 static_assert(kPersistMaxGroups <= xpbd::kPatchThreads, "mailbox readers");
 // XPBD_PATH_AUTO takes the persistent substep up to this many groups.  Per
 // iteration on resting boxes (persistent against the graph of phased
-// kernels): 24 groups 10.8-11.7 against 16.0 us, 47 groups 14.3 against 16.1,
-// 81 groups 16.5 against 16.7, 116 groups 19.2 against 16.6.
-constexpr int kAutoPersistGroups = 48;
+// kernels): 24 groups 10.9-11.2 against 16.0 us, 47 groups 13.3 against 16.1,
+// 81 groups 15.0 against 16.6, 116 groups 16.2 against 16.6, 129 groups 16.2
+// against 17.1, 159 groups 17.6 against 17.6; the 32^3 twisted grid (192
+// groups) 17.1 against 17.0.
+constexpr int kAutoPersistGroups = 144;
 // A batch takes the persistent substep automatically up to this many
 // workgroups in all (every member within kAutoPersistGroups).  With the scene
 // barriers free of L2 write-backs and invalidates, a batch of small members

@@ -3,7 +3,7 @@
 A `SceneBatch` concatenates the patches and vertex blocks of its member
 `XPBDNeoHookeanSolver` scenes into one work list, so each kernel of a substep
 (and one replayed hipGraph per substep or per frame) covers every scene.
-Batches of small scenes (each at most 96 patches, 512 in all) take the
+Batches of small scenes (each at most 288 patches, 512 in all) take the
 persistent substep instead: one launch per substep, in which each member's
 workgroups meet at that member's own barriers (option `persistent`).
 Scenes keep their own buffers, materials, SDFs and rho/omega state; the

@@ -110,7 +110,7 @@ def test_large_scene_falls_back_to_graph():
     (same bits)."""
     from scenes import jittered_box
     print("[case] persistent fallback", flush=True)
-    scene = jittered_box(n=24, frames=1, iters=10)
+    scene = jittered_box(n=38, frames=1, iters=10)  # 643 patches
     a, b = _pair(scene, 1)
     _same(a, b)
 
@@ -204,17 +204,33 @@ def test_batch_persistent_path_selection():
         assert b.path == "graph"
     with SceneBatch(small, persistent="never", graph=False) as b:
         assert b.path == "kernels"
-    # a member of more than 96 patches: automatic selection keeps the phased
-    # kernels, "always" takes the persistent substep while all members fit
+    # a member of more than 512 patches: the persistent substep does not apply
+    # (automatic selection keeps the phased kernels, "always" refuses)
+    huge = make_solver(batched_scenes(n_scenes=1, seed=5, n_range=(38, 38), iters=5)[0])
+    with SceneBatch([huge], persistent="never") as b:
+        assert b.n_patches > 512, b.n_patches
+    with SceneBatch([huge] + small) as b:
+        assert b.path == "graph"
+    with SceneBatch([huge] + small, persistent="always") as b:
+        with pytest.raises(ValueError):
+            b.substep(1)
+    # a member of 289-512 patches: automatic selection keeps the phased
+    # kernels, "always" takes the persistent substep
+    mid = make_solver(batched_scenes(n_scenes=1, seed=5, n_range=(30, 30), iters=5)[0])
+    with SceneBatch([mid], persistent="never") as b:
+        assert 288 < b.n_patches <= 512, b.n_patches
+    with SceneBatch([mid] + small) as b:
+        assert b.path == "graph"
+    with SceneBatch([mid] + small, persistent="always") as b:
+        assert b.path == "persistent"
+    # a member of 97-288 patches takes it automatically, with the same bits
     big_scene = batched_scenes(n_scenes=1, seed=5, n_range=(22, 22), iters=5)[0]
     big = make_solver(big_scene)
     with SceneBatch([big], persistent="never") as b:
         groups = (b.n_patches + 1) // 2
     assert 48 < groups <= 128, groups
-    with SceneBatch([big] + small) as b:
-        assert b.path == "graph"
     ref = make_solver(big_scene, exec_path="graph")
-    with SceneBatch([big] + small, persistent="always") as b:
+    with SceneBatch([big] + small) as b:
         assert b.path == "persistent"
         b.substep(2)
         b.synchronize()
