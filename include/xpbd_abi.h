@@ -296,8 +296,8 @@ int xpbd_batch_set_members(xpbd_batch* b, xpbd_solver* const* scenes, int32_t n_
  * device allocation or release.  Limits: at most XPBD_NOALLOC_MAX_SCENES
  * members and XPBD_NOALLOC_LIST_BYTES of work lists (32 B per patch of 512
  * tets, 8 B per vertex block and per persistent workgroup, 16 B per
- * entry of the persistent substep's per-XCD workgroup list, at most 256
- * entries); larger member
+ * entry of the persistent substep's workgroup list: one per workgroup, or
+ * 8 per row when the members are laid out per XCD); larger member
  * lists are refused with XPBD_EINVAL and leave the batch as it was.  The
  * batch must not replay graphs (opts.graph == 0: capturing a new one would
  * allocate).  The array holds one batch's lists per GPU: a later call for
