@@ -18,10 +18,14 @@ cases are the reference's convergence-comparison preset
 the convergence monitor on, one run per method: split without and with
 Chebyshev, block without and with Chebyshev) on the demo's cow from that same
 frame-54 state, 3 frames each; they take the mesh and starting state from the
-cow_demo_contact fixture instead of storing them again.
+cow_demo_contact fixture instead of storing them again.  The two
+`armadillo_compare_*` cases run that preset on its own mesh
+(assets/tetmesh/armadillo.node at scale 0.01) from the state of its block +
+Chebyshev run after 60 frames, when it rests on the sphere: block with
+Chebyshev, and split without, 2 frames each.
 
 Usage (from the repository root; reads the reference checkout only for the
-cow mesh):  python tests/golden/make_golden.py [--reference /root/reference]
+cow and armadillo meshes):  python tests/golden/make_golden.py [--reference /root/reference]
 """
 from __future__ import annotations
 
@@ -131,6 +135,23 @@ def cases(reference: str):
             out["cow_demo_contact"], frames=3, base="cow_demo_contact",
             params=dict(demo, block_neohookean_enable=block, cheb_enable=cheb,
                         monitor_convergence=True))
+    # the same preset on its own mesh: the armadillo at scale 0.01, from the
+    # state of the block + Chebyshev run after 60 frames (resting on the
+    # sphere), 2 frames per method; block + Chebyshev (the flagship method)
+    # and split without Chebyshev (the reference's baseline method)
+    arm = read_tetgen(os.path.join(reference, "assets", "tetmesh", "armadillo.node"))
+    preset = dict(demo, scale=0.01, monitor_convergence=True)
+    warm = Oracle(arm.verts, arm.tets, sdfs, precision=32, threads=8, **preset)
+    warm.compute_lumped_mass(1.0)
+    warm.set_log(False)
+    warm.frame(60)
+    out["armadillo_compare_block_dc"] = dict(
+        verts=arm.verts, tets=arm.tets, x0=warm.x.astype(np.float32),
+        v0=warm.v.astype(np.float32), rho0=float(warm.rho_hat), sdfs=sdfs,
+        gravity=(0.0, -9.8, 0.0), total_mass=1.0, frames=2, params=preset)
+    out["armadillo_compare_nh"] = dict(
+        out["armadillo_compare_block_dc"], base="armadillo_compare_block_dc",
+        params=dict(preset, block_neohookean_enable=False, cheb_enable=False))
     return out
 
 
