@@ -605,6 +605,65 @@ def small_configs_measurement(frames: int = 10) -> dict:
     return out
 
 
+def presets_measurement(frames: int = 301) -> dict:
+    """The reference's two scripted presets, each through the command-line
+    driver's solver set-up (run_neohookean.make_solver, the same flags) on its
+    own TetGen mesh (taken from the golden fixtures, which hold the reference's
+    meshes), on the default execution path:
+    scripts/demo.bat: the cow at scale 1.5, E 5e8, nu 0.4999, dt 0.01 at
+      60 fps (two time chunks per frame), one substep of 200 iterations,
+      block + Chebyshev, 301 frames;
+    scripts/compare_convergence_armadillo.bat: the armadillo at scale 0.01 with
+      the same parameters and the convergence monitor on, 301 frames for each
+      of its four methods (split and block, each without and with Chebyshev).
+    Positions are read back after every frame, as the driver's USD writer
+    does; USD, plots and reports are not written."""
+    import tempfile
+
+    import run_neohookean
+    from xpbd_gpu.mesh import TetMesh, write_tetgen
+
+    golden = os.path.join(ROOT, "tests", "golden")
+    common = ["--young", "500000000", "--dt", "0.01", "--fps", "60", "--frames", str(frames),
+              "--substeps", "1", "--iterations", "200", "--no-usd"]
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        models = {}
+        for name, fixture in (("cow", "cow_demo_contact"), ("armadillo",
+                                                           "armadillo_compare_block_dc")):
+            with np.load(os.path.join(golden, fixture + ".npz"), allow_pickle=False) as z:
+                write_tetgen(os.path.join(d, name), TetMesh(z["verts"], z["tets"]))
+            models[name] = os.path.join(d, name + ".node")
+        runs = [("demo_cow", ["--model", models["cow"], "--scale", "1.5", "--block-neohookean",
+                              "--cheb"])]
+        for tag, flags in (("NH", ["--split-neohookean", "--no-cheb"]),
+                           ("NH-DC", ["--split-neohookean", "--cheb"]),
+                           ("BNH", ["--block-neohookean", "--no-cheb"]),
+                           ("BNH-DC", ["--block-neohookean", "--cheb"])):
+            runs.append(("armadillo_" + tag, ["--model", models["armadillo"], "--scale", "0.01",
+                                              "--monitor-convergence", *flags]))
+        for name, argv in runs:
+            args = run_neohookean.build_parser().parse_args(argv + common)
+            solver, _ = run_neohookean.make_solver(args)
+            t0 = time.perf_counter()
+            for _ in range(frames):
+                solver.solve()
+                solver.mesh.verts.x.to_numpy()
+            el = time.perf_counter() - t0
+            solver.check_finite()
+            n_tets = int(solver.n_tets)
+            # dt 0.01 at 60 fps: each frame is a 0.01 s and a 1/150 s chunk
+            it_per_frame = 2 * args.substeps * args.iterations
+            out[name] = {"tets": n_tets, "frames": frames, "seconds": el,
+                         "ms_per_frame": el / frames * 1e3,
+                         "iterations_per_frame": it_per_frame,
+                         "tet_iterations_per_s": n_tets * it_per_frame * frames / el}
+            print(f"[bench] preset {name}: {n_tets} tets, {frames} frames in {el:.2f} s",
+                  file=sys.stderr, flush=True)
+            solver.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -641,6 +700,8 @@ def main():
                     help="only summarize a rocprofv3 --kernel-trace --stats folder")
     ap.add_argument("--timing-replays", type=int, default=8,
                     help="back-to-back projection replays per event pair for the roofline")
+    ap.add_argument("--presets-only", action="store_true",
+                    help="time only the reference's scripted presets (demo, armadillo comparison)")
     ap.add_argument("--small-only", action="store_true",
                     help="only measure the small single-scene configurations, print JSON")
     ap.add_argument("--summarize-small-stats", default="",
@@ -687,6 +748,10 @@ def main():
 
     if args.small_only:
         print(json.dumps(small_configs_measurement()), flush=True)
+        return
+
+    if args.presets_only:
+        print(json.dumps(presets_measurement()), flush=True)
         return
 
     if args.batched_only:
@@ -781,6 +846,7 @@ def main():
     del solver
     if not args.plain_only:
         out["small_configs"] = small_configs_measurement()
+        out["presets"] = presets_measurement()
         out["batched"] = batched_measurement(args.batched_scenes, args.batched_frames,
                                              not args.no_tune)
         out["served"] = served_measurement(args.serve_scenes, args.serve_rate,

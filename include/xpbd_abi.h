@@ -206,8 +206,8 @@ int xpbd_get_sizes(xpbd_solver* s, int32_t* n_verts, int32_t* n_tets);
  *  XPBD_PATH_KERNELS:    one launch per kernel per iteration.
  *  XPBD_PATH_GRAPH:      the same kernels, each substep captured in a hipGraph.
  *  XPBD_PATH_PERSISTENT: for small scenes (at most 512 patches of 512 tets,
- *                        the patch layout with Chebyshev on and no isolated
- *                        vertices, at least one iteration), one launch per
+ *                        the patch layout without isolated vertices, at
+ *                        least one iteration), one launch per
  *                        substep that runs every iteration with scene-wide
  *                        waits between its phases; bit-identical to the
  *                        other paths in deterministic mode.  Other scenes take

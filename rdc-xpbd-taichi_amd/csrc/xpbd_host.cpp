@@ -996,13 +996,13 @@ constexpr int kAutoPersistGroups = 144;
 // 238, against 14.8-17.3 us for the phased launches.
 constexpr int kAutoBatchGroups = 256;
 
-// Whether the persistent substep kernel can run this scene: the fused
-// sequence with at least one iteration, and few enough groups to be resident
-// together.
+// Whether the persistent substep kernel can run this scene: the patch layout
+// without isolated vertices (Chebyshev on or off), at least one iteration,
+// and few enough groups to be resident together.
 bool persistent_ok(const xpbd_solver* s) {
   const SceneDev& S = s->host_scene;
   const int groups = S.n_partials;
-  return s->layout == XPBD_LAYOUT_PATCH && S.fused && S.iters >= 1 && groups >= 1 &&
+  return s->layout == XPBD_LAYOUT_PATCH && !s->has_isolated && S.iters >= 1 && groups >= 1 &&
          groups <= kPersistMaxGroups && groups <= s->persist_capacity;
 }
 
@@ -2135,8 +2135,9 @@ int batch_run(xpbd_batch* b, int n_substeps) {
   }
   if (b->opts.persistent == 2)
     return set_error(XPBD_EINVAL, "the persistent substep does not apply to this batch (patch "
-                                  "layout, fused sequence, at most %d groups per member and %d "
-                                  "in all)", kPersistMaxGroups, b->persist_capacity);
+                                  "layout without isolated vertices, at least one iteration, at "
+                                  "most %d groups per member and %d in all)", kPersistMaxGroups,
+                     b->persist_capacity);
   if (!b->opts.graph) {
     for (int k = 0; k < n_substeps; ++k) {
       TRY(batch_enqueue_substep(b));

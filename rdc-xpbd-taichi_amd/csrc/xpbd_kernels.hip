@@ -1131,8 +1131,9 @@ __device__ __forceinline__ bool mailbox_sums(const SceneDev& S, int it, bool all
 // barrier after the slot partials are written, and one wait for the residual
 // partials, which arrive by value in the scene's mailbox.  Every sum is taken
 // in the phased kernels' order, so the results are bit-identical to them in
-// deterministic mode.  Requires the fused
-// sequence (patch layout, Chebyshev on, no isolated vertices), at least one
+// deterministic mode.  With Chebyshev off the next iterate is x_J itself,
+// and the residual pass and its wait run only for the convergence monitor.
+// Requires the patch layout without isolated vertices, at least one
 // iteration, and all of the launch's workgroups resident at once.
 //
 // kBatch = false: one scene (scenes[0]), workgroup g = residual group g, and
@@ -1232,28 +1233,40 @@ __global__ void __launch_bounds__(kPersistThreads)
   unsigned barriers = 0;
   bool ok = true;
   const bool writer = group == 0;
+  // Chebyshev on: the blend folded into the iteration (the fused sequence);
+  // off: the next iterate is x_J itself (post_solve writes pred_x).  The
+  // residual pass runs for Chebyshev or the convergence monitor, as in the
+  // phased kernels.
+  const bool fused = S.cheb != 0;
+  const bool need_res = S.cheb || S.monitor;
   for (int it = 0; it < S.iters && ok; ++it) {
     if (it > 0) {
-      // rho/omega update after iteration it - 1 from every group's partial,
-      // as each arrives in the mailbox
-      double tot[3];
-      ok = mailbox_sums(S, it - 1, S.monitor != 0, tot, s_red, err, timeout, s_wok);
-      if (!ok) break;
-      if (tid == 0) {
-        double L[6];
-        const SceneCtrl c = ctrl_update(S, s_ctrl, it - 1, tot, L);
-        s_ctrl = c;
-        s_omega = float(c.omega);
-        if (writer) {
-          S.ctrl[(it - 1) & 1] = c;
-          store_log_row(S, it - 1, L);
+      if (need_res) {
+        // rho/omega update (and the log row) after iteration it - 1 from
+        // every group's partial, as each arrives in the mailbox
+        double tot[3];
+        ok = mailbox_sums(S, it - 1, S.monitor != 0, tot, s_red, err, timeout, s_wok);
+        if (!ok) break;
+        if (tid == 0) {
+          double L[6];
+          const SceneCtrl c = ctrl_update(S, s_ctrl, it - 1, tot, L);
+          s_ctrl = c;
+          s_omega = float(c.omega);
+          if (writer) {
+            S.ctrl[(it - 1) & 1] = c;
+            store_log_row(S, it - 1, L);
+          }
         }
+        __syncthreads();
       }
-      __syncthreads();
-      if (lt < h.n_verts) {  // y^it from y^(it-1), x_J^(it-1) and y^(it-2)
-        const float4 y = s_y[half][lt];
-        s_y[half][lt] = cheb_blend(S, y, s_x[half][lt], s_prev[half][lt], s_omega);
-        s_prev[half][lt] = y;
+      if (lt < h.n_verts) {
+        if (fused) {  // y^it from y^(it-1), x_J^(it-1) and y^(it-2)
+          const float4 y = s_y[half][lt];
+          s_y[half][lt] = cheb_blend(S, y, s_x[half][lt], s_prev[half][lt], s_omega);
+          s_prev[half][lt] = y;
+        } else {
+          s_y[half][lt] = s_x[half][lt];
+        }
       }
       __syncthreads();
     }
@@ -1319,62 +1332,64 @@ __global__ void __launch_bounds__(kPersistThreads)
     if (!ok) break;
     // Every group has taken the partials of iteration it - 1 before arriving
     // here, so this group's words of that parity are emptied for it + 1.
-    if (tid < 3)
+    if (need_res && tid < 3)
       __hip_atomic_store((gptr<unsigned long long>)S.mailbox +
                              (size_t((it + 1) & 1) * 3 + tid) * G + group,
                          kMailboxEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // Jacobi average of the patch's vertices from every slot of each
     if (lt < h.n_verts) s_x[half][lt] = jacobi_from(S, slot_sum<true>(S, sb, se), val, s_y[half][lt]);
     __syncthreads();
-    // residual at x_J: lane sums over the group's two patches in order
-    float rv[kPatchTetsPerThread], hv[kPatchTetsPerThread], dv[kPatchTetsPerThread];
-#pragma unroll
-    for (int j = 0; j < kPatchTetsPerThread; ++j) {
-      rv[j] = hv[j] = dv[j] = 0.f;
-      if (lt + j * kPatchThreads < h.n_tets) {
-        const uint2 loc = t.loc[j];
-        const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu),
-                          int((loc.x >> 16) & 0xffu), int(loc.x >> 24)};
-        residual_tet(S, f3(s_x[half][l[0]]), f3(s_x[half][l[1]]), f3(s_x[half][l[2]]),
-                     f3(s_x[half][l[3]]), idm[j], inv_det[j], t.lam[j], rv[j], hv[j], dv[j]);
-      }
-    }
-    // the second patch's terms go to LDS (the corner array is free now)
-    float* xch = reinterpret_cast<float*>(s_sorted[0]);
-    if (half == 1) {
+    if (need_res) {
+      // residual at x_J: lane sums over the group's two patches in order
+      float rv[kPatchTetsPerThread], hv[kPatchTetsPerThread], dv[kPatchTetsPerThread];
 #pragma unroll
       for (int j = 0; j < kPatchTetsPerThread; ++j) {
-        xch[(3 * j + 0) * kPatchThreads + lt] = rv[j];
-        xch[(3 * j + 1) * kPatchThreads + lt] = hv[j];
-        xch[(3 * j + 2) * kPatchThreads + lt] = dv[j];
+        rv[j] = hv[j] = dv[j] = 0.f;
+        if (lt + j * kPatchThreads < h.n_tets) {
+          const uint2 loc = t.loc[j];
+          const int l[4] = {int(loc.x & 0xffu), int((loc.x >> 8) & 0xffu),
+                            int((loc.x >> 16) & 0xffu), int(loc.x >> 24)};
+          residual_tet(S, f3(s_x[half][l[0]]), f3(s_x[half][l[1]]), f3(s_x[half][l[2]]),
+                       f3(s_x[half][l[3]]), idm[j], inv_det[j], t.lam[j], rv[j], hv[j], dv[j]);
+        }
       }
-    }
-    __syncthreads();
-    float acc_r = 0.f, acc_h = 0.f, acc_d = 0.f;
-    if (half == 0) {
+      // the second patch's terms go to LDS (the corner array is free now)
+      float* xch = reinterpret_cast<float*>(s_sorted[0]);
+      if (half == 1) {
 #pragma unroll
-      for (int j = 0; j < kPatchTetsPerThread; ++j) {
-        acc_r += rv[j];
-        acc_h += hv[j];
-        acc_d += dv[j];
+        for (int j = 0; j < kPatchTetsPerThread; ++j) {
+          xch[(3 * j + 0) * kPatchThreads + lt] = rv[j];
+          xch[(3 * j + 1) * kPatchThreads + lt] = hv[j];
+          xch[(3 * j + 2) * kPatchThreads + lt] = dv[j];
+        }
       }
+      __syncthreads();
+      float acc_r = 0.f, acc_h = 0.f, acc_d = 0.f;
+      if (half == 0) {
 #pragma unroll
-      for (int j = 0; j < kPatchTetsPerThread; ++j) {
-        acc_r += xch[(3 * j + 0) * kPatchThreads + lt];
-        acc_h += xch[(3 * j + 1) * kPatchThreads + lt];
-        acc_d += xch[(3 * j + 2) * kPatchThreads + lt];
+        for (int j = 0; j < kPatchTetsPerThread; ++j) {
+          acc_r += rv[j];
+          acc_h += hv[j];
+          acc_d += dv[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kPatchTetsPerThread; ++j) {
+          acc_r += xch[(3 * j + 0) * kPatchThreads + lt];
+          acc_h += xch[(3 * j + 1) * kPatchThreads + lt];
+          acc_d += xch[(3 * j + 2) * kPatchThreads + lt];
+        }
       }
+      double tot[3];
+      block_sums(acc_r, acc_h, acc_d, S.monitor != 0, tot);
+      if (tid == 0) post_partials(S, it, group, tot);
     }
-    double tot[3];
-    block_sums(acc_r, acc_h, acc_d, S.monitor != 0, tot);
-    if (tid == 0) post_partials(S, it, group, tot);
   }
   double tot[3];
-  if (ok) ok = mailbox_sums(S, S.iters - 1, true, tot, s_red, err, timeout, s_wok);
+  if (ok && need_res) ok = mailbox_sums(S, S.iters - 1, true, tot, s_red, err, timeout, s_wok);
   if (ok) {
     // the update after the last iteration (k_reduce), the final blend
-    // (k_final_blend) and advance (k_advance)
-    if (tid == 0) {
+    // (k_final_blend) or x_J itself, and advance (k_advance)
+    if (tid == 0 && need_res) {
       double L[6];
       const SceneCtrl c = ctrl_update(S, s_ctrl, S.iters - 1, tot, L);
       s_omega = float(c.omega);
@@ -1386,9 +1401,11 @@ __global__ void __launch_bounds__(kPersistThreads)
     }
     __syncthreads();
     if (lt < h.n_verts && vs.x < 0) {  // the vertex's owner patch writes it
-      const float4 pf = cheb_blend(S, s_y[half][lt], s_x[half][lt], s_prev[half][lt], s_omega);
+      const float4 pf =
+          fused ? cheb_blend(S, s_y[half][lt], s_x[half][lt], s_prev[half][lt], s_omega)
+                : s_x[half][lt];
       S.pos[g] = pf;
-      S.xj[g] = s_x[half][lt];
+      if (fused) S.xj[g] = s_x[half][lt];
       advance_vertex(S, g, pf, xs0);
     }
 #pragma unroll

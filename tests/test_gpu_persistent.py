@@ -72,6 +72,48 @@ def test_persistent_split_mode_and_monitor():
     assert ha == hb
 
 
+@pytest.mark.parametrize("block,monitor", [(True, False), (True, True), (False, True),
+                                           (False, False)])
+def test_persistent_without_chebyshev(block, monitor):
+    """Chebyshev off (the reference's NH and BNH methods): the persistent
+    substep takes x_J as the next iterate and runs the residual pass only for
+    the convergence monitor; the same bits as the graph.  A wait forced to
+    give up is reported, so the persistent path is the one that ran."""
+    from xpbd_gpu._abi import XpbdError
+    print(f"[case] persistent no-cheb block={block} monitor={monitor}", flush=True)
+    scene = resting_cube(frames=2)  # 48 patches: 24 workgroups
+    scene.params.update(cheb_enable=False, block_neohookean_enable=block,
+                        monitor_convergence=monitor)
+    a, b = _pair(scene, 2)
+    _same(a, b)
+    if monitor:
+        ha = [(r["hydro_error"], r["dev_error"]) for r in a.convergence_history]
+        hb = [(r["hydro_error"], r["dev_error"]) for r in b.convergence_history]
+        assert len(ha) == 2 * 5 * 30 and ha == hb
+    b.set_barrier_timeout(0)
+    with pytest.raises(XpbdError, match="timed out"):
+        b.solve()
+
+
+def test_batch_persistent_without_chebyshev():
+    print("[case] batch persistent no-cheb", flush=True)
+    scenes = _mixed_batch_scenes()
+    for sc in scenes:
+        sc.params.update(cheb_enable=False, monitor_convergence=True)
+    alone = [make_solver(s, exec_path="graph") for s in scenes]
+    per = [make_solver(s) for s in scenes]
+    for s in alone:
+        s.solve()
+    with SceneBatch(per) as b:
+        assert b.path == "persistent"
+        b.frame(1)
+        b.synchronize()
+    for a, p in zip(alone, per):
+        assert np.array_equal(a.mesh.verts.x.to_numpy(), p.mesh.verts.x.to_numpy())
+        assert np.array_equal(a.mesh.verts.v.to_numpy(), p.mesh.verts.v.to_numpy())
+        assert np.array_equal(a.mesh.cells.lambda_D.to_numpy(), p.mesh.cells.lambda_D.to_numpy())
+
+
 def test_persistent_reference_demo():
     print("[case] persistent cow demo", flush=True)
     case = golden_cases.load("cow_demo_contact")
