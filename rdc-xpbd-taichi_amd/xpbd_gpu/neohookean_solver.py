@@ -16,9 +16,9 @@ Execution options with no reference counterpart are keyword-only:
                    coordinate bisection; True, the default), "morton" or
                    "none" (the caller's order; False)
   device           HIP device ordinal
-  exec_path        "auto" (default: "persistent" for scenes of at most 96
+  exec_path        "auto" (default: "persistent" for scenes of at most 288
                    patches, "graph" otherwise), "graph" (hipGraph per substep),
-                   "persistent" (one launch per substep, scenes of at most 256
+                   "persistent" (one launch per substep, scenes of at most 512
                    patches) or "kernels"
   layout           "patch" (workgroup per LDS-staged patch, default) or "flat"
   log_residuals    keep r_hat / rho_hat / omega per iteration in
