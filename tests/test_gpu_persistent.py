@@ -51,11 +51,14 @@ def test_persistent_matches_graph_bitwise(name, scene):
     _same(a, b)
 
 
-@pytest.mark.parametrize("iters", [1, 2, 5, 6, 40])
-def test_persistent_iteration_counts(iters):
-    print(f"[case] persistent iters {iters}", flush=True)
+@pytest.mark.parametrize("iters,cheb", [(1, True), (2, True), (5, True), (6, True), (40, True),
+                                        (1, False), (2, False)])
+def test_persistent_iteration_counts(iters, cheb):
+    print(f"[case] persistent iters {iters} cheb {cheb}", flush=True)
     scene = resting_cube(n=6, frames=2)
     scene.params["constraints_iter"] = iters
+    scene.params["cheb_enable"] = cheb
+    scene.params["monitor_convergence"] = not cheb
     a, b = _pair(scene, 2)
     _same(a, b)
 
@@ -139,12 +142,15 @@ def test_persistent_hang_and_cube_colliders():
     _same(a, b)
 
 
-def test_persistent_atomic_mode_matches_oracle():
-    print("[case] persistent atomic", flush=True)
+@pytest.mark.parametrize("cheb", [True, False])
+def test_persistent_atomic_mode_matches_oracle(cheb):
+    print(f"[case] persistent atomic cheb {cheb}", flush=True)
     scene = resting_cube(frames=5)
+    scene.params["cheb_enable"] = cheb
     solver, orc = run_pair(scene, accumulate_mode="atomic", exec_path="persistent")
     check_positions(scene, solver, orc)
-    check_residual_curve(solver, orc)
+    if cheb:
+        check_residual_curve(solver, orc)
 
 
 def test_large_scene_falls_back_to_graph():
