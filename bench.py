@@ -271,7 +271,10 @@ def summarize_small_stats(folder: str) -> str:
         "# and the 32^3 twisted grid (196 608 tets, 200 iterations per frame); each 1 untimed +",
         "# 10 timed frames on the default path (auto: the cube's substeps run as one",
         "# k_substep_persistent launch each, the grid as phased kernels) and again on the graph",
-        "# path (phased kernels for both).  Kernels of all runs are pooled per name.",
+        "# path (phased kernels for both); then batches of 2, 4, 8 and 14 copies of the 14^3",
+        "# cube (5 x 30 iterations per frame, 1 untimed + 5 timed frames) on the batched",
+        "# persistent substep (k_substep_persistent<true>) and on the phased kernels",
+        "# (k_patch_project<true>).  Kernels of all runs are pooled per name.",
         "# Made by: " + SMALL_STATS_COMMAND])
 
 
@@ -605,6 +608,36 @@ def small_configs_measurement(frames: int = 10) -> dict:
     return out
 
 
+def small_batch_measurement(sizes=(2, 4, 8, 14), frames: int = 5) -> dict:
+    """Batches of small scenes on the persistent substep against the phased
+    kernels (graph): k copies of the 14^3 cube resting on the plane (16 464
+    tets, 33 patches, 17 workgroups each), 5 substeps x 30 iterations per
+    frame, one untimed frame, then `frames` timed frames per path."""
+    from xpbd_gpu.batch import SceneBatch
+    from xpbd_gpu.scenes import make_solver, resting_cube
+
+    out = {}
+    for k in sizes:
+        row = {"scenes": k}
+        for per in ("always", "never"):
+            sols = [make_solver(resting_cube(n=14, frames=frames + 1)) for _ in range(k)]
+            with SceneBatch(sols, persistent=per) as b:
+                b.frame(1)
+                b.synchronize()
+                t0 = time.perf_counter()
+                b.frame(frames)
+                b.synchronize()
+                el = time.perf_counter() - t0
+                it = frames * sols[0].substeps_num * sols[0].constraints_iter
+                row["patches"] = b.n_patches
+                row[b.path + "_us_per_iteration"] = el / it * 1e6
+            for s in sols:
+                s.close()
+        out[f"{k}_scenes"] = row
+        print(f"[bench] small batch {k} x cube14: {row}", file=sys.stderr, flush=True)
+    return out
+
+
 def presets_measurement(frames: int = 301) -> dict:
     """The reference's two scripted presets, each through the command-line
     driver's solver set-up (run_neohookean.make_solver, the same flags) on its
@@ -747,7 +780,8 @@ def main():
         sys.exit(2)
 
     if args.small_only:
-        print(json.dumps(small_configs_measurement()), flush=True)
+        print(json.dumps({"small_configs": small_configs_measurement(),
+                          "small_batches": small_batch_measurement()}), flush=True)
         return
 
     if args.presets_only:
@@ -846,6 +880,7 @@ def main():
     del solver
     if not args.plain_only:
         out["small_configs"] = small_configs_measurement()
+        out["small_batches"] = small_batch_measurement()
         out["presets"] = presets_measurement()
         out["batched"] = batched_measurement(args.batched_scenes, args.batched_frames,
                                              not args.no_tune)
