@@ -1042,59 +1042,75 @@ __device__ __forceinline__ bool scene_barrier(gptr<unsigned> sync, gptr<unsigned
   return *s_ok != 0;
 }
 
+// One wave's share of a mailbox wait: lane l polls residual group w0 + l's
+// words of iteration `it` (components 0, and 1-2 when `all3`) with
+// agent-scope atomic loads until none of the wave's is empty, so the values
+// arrive with the wait, then sums them over the wave as scene_sums does (each
+// lane's entry added to 0.0, then the lane-exchange butterfly): every lane
+// gets the wave's sums.  Lanes without an entry add nothing.  The wave gives
+// up when the error word is set, or sets it after `timeout` wall-clock ticks;
+// returns 0 then (uniform over the wave), else 1.
+__device__ __forceinline__ int mailbox_wave(const SceneDev& S, int it, bool all3, int w0,
+                                            double acc[3], gptr<unsigned> err,
+                                            unsigned long long timeout) {
+  const int G = S.n_partials;
+  const int e = w0 + int(threadIdx.x & 63);
+  gptr<const unsigned long long> box =
+      (gptr<const unsigned long long>)(S.mailbox + size_t(it & 1) * 3 * size_t(G));
+  const int nc = all3 ? 3 : 1;
+  unsigned long long u[3] = {0ull, 0ull, 0ull};
+  int ok = 1;
+  const unsigned long long t0 = wall_clock64();
+  while (true) {
+    bool have = true;
+    if (e < G) {
+      for (int c = 0; c < nc; ++c) {
+        u[c] = __hip_atomic_load(box + size_t(c) * G + e, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        have = have && u[c] != kMailboxEmpty;
+      }
+    }
+    if (__all(have)) break;
+    if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
+      ok = 0;
+      break;
+    }
+    if (wall_clock64() - t0 > timeout) {
+      if ((threadIdx.x & 63) == 0)
+        __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ok = 0;
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    acc[c] = 0.0;
+    if (c > 0 && !all3) continue;
+    double sum = 0.0;
+    if (e < G) sum += __builtin_bit_cast(double, u[c]);
+    acc[c] = wave_sum_d(sum);
+  }
+  return ok;
+}
+
 // The sums of every residual group's partials of iteration `it` (component 0,
-// and 1-2 when `all3`), waiting for each to be posted in the mailbox.  Thread
-// t < n_partials (at most kReduceThreads) polls group t's words with
-// agent-scope atomic loads until none is empty, so the values arrive with the
-// wait; the sums are then taken exactly as scene_sums takes them (thread t
-// holds entry t, then the wave sums, then the waves in order), so the same
-// bits.  A wave gives up when the error word is set, or sets it after
-// `timeout` wall-clock ticks.  Returns false on either (to every thread).
-// Thread 0 gets the totals.  Ends with a workgroup barrier.
+// and 1-2 when `all3`), waiting for each to be posted in the mailbox: wave w
+// of the first kReduceThreads threads takes groups 64w..64w+63
+// (mailbox_wave), then thread 0 adds the waves' sums in order, so the sums
+// are those of scene_sums (thread t holding entry t), the same bits.  Returns
+// false to every thread when a wave gave up.  Thread 0 gets the totals.  Ends
+// with a workgroup barrier.
 __device__ __forceinline__ bool mailbox_sums(const SceneDev& S, int it, bool all3, double tot[3],
                                              double* red, gptr<unsigned> err,
                                              unsigned long long timeout, int* s_wok) {
   constexpr int kW = kReduceThreads / 64;
   const int G = S.n_partials;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double acc[3] = {0.0, 0.0, 0.0};
   if (tid < kReduceThreads) {
+    double acc[3] = {0.0, 0.0, 0.0};
     int ok = 1;
-    if (wave * 64 < G) {
-      gptr<const unsigned long long> box =
-          (gptr<const unsigned long long>)(S.mailbox + size_t(it & 1) * 3 * size_t(G));
-      const int nc = all3 ? 3 : 1;
-      unsigned long long u[3] = {0ull, 0ull, 0ull};
-      const unsigned long long t0 = wall_clock64();
-      while (true) {
-        bool have = true;
-        if (tid < G) {
-          for (int c = 0; c < nc; ++c) {
-            u[c] = __hip_atomic_load(box + size_t(c) * G + tid, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-            have = have && u[c] != kMailboxEmpty;
-          }
-        }
-        if (__all(have)) break;
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) {
-          ok = 0;
-          break;
-        }
-        if (wall_clock64() - t0 > timeout) {
-          if (lane == 0) __hip_atomic_fetch_or(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ok = 0;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        if (c > 0 && !all3) break;
-        double sum = 0.0;
-        if (tid < G) sum += __builtin_bit_cast(double, u[c]);
-        acc[c] = wave_sum_d(sum);
-      }
-    }
+    if (wave * 64 < G) ok = mailbox_wave(S, it, all3, wave * 64, acc, err, timeout);
     if (lane == 0) {
       s_wok[wave] = ok;
       red[wave] = acc[0];
@@ -1244,10 +1260,7 @@ __global__ void __launch_bounds__(kPersistThreads)
       if (need_res) {
         // rho/omega update (and the log row) after iteration it - 1 from
         // every group's partial, as each arrives in the mailbox
-        double tot[3];
-        ok = mailbox_sums(S, it - 1, S.monitor != 0, tot, s_red, err, timeout, s_wok);
-        if (!ok) break;
-        if (tid == 0) {
+        auto update = [&](const double tot[3]) {
           double L[6];
           const SceneCtrl c = ctrl_update(S, s_ctrl, it - 1, tot, L);
           s_ctrl = c;
@@ -1256,8 +1269,38 @@ __global__ void __launch_bounds__(kPersistThreads)
             S.ctrl[(it - 1) & 1] = c;
             store_log_row(S, it - 1, L);
           }
+        };
+        if (G <= 64) {
+          // One wave holds every group, and the other waves of mailbox_sums
+          // would add exact zeros: wave 0 waits and sums alone, thread 0
+          // adds the waves' sums in the same order and updates the
+          // controller, and one workgroup barrier follows instead of three.
+          if (tid < 64) {
+            double acc[3];
+            const int ok0 = mailbox_wave(S, it - 1, S.monitor != 0, 0, acc, err, timeout);
+            if (tid == 0) {
+              s_ok = ok0;
+              if (ok0) {
+                double tot[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                  tot[c] = 0.0;
+                  for (int w = 0; w < kReduceThreads / 64; ++w) tot[c] += w == 0 ? acc[c] : 0.0;
+                }
+                update(tot);
+              }
+            }
+          }
+          __syncthreads();
+          ok = s_ok != 0;
+          if (!ok) break;
+        } else {
+          double tot[3];
+          ok = mailbox_sums(S, it - 1, S.monitor != 0, tot, s_red, err, timeout, s_wok);
+          if (!ok) break;
+          if (tid == 0) update(tot);
+          __syncthreads();
         }
-        __syncthreads();
       }
       if (lt < h.n_verts) {
         if (fused) {  // y^it from y^(it-1), x_J^(it-1) and y^(it-2)
